@@ -1,0 +1,413 @@
+// ref_driver.cpp - C-ABI shim over the reference's own gravity headers.
+//
+// TEST INFRASTRUCTURE ONLY.  Built by oracle/Makefile (target `ref`) into
+// oracle/_ref/libvipp_ref.so from the reference sources where they lie
+// (gravity/include of the reference repository, given by -I).  Nothing of the
+// reference is copied here: this file only instantiates the reference's
+// templates in double precision and reads their results through the friend
+// hooks the reference declares for its tests under ENABLE_TEST
+// (octree.hpp:123-125, fmm.hpp:495-497, solver_ref.hpp:29-32,
+// solver_ref_naive.hpp:27-29, problem.hpp:245-247, benchmark.hpp:177-179).
+//
+// The reference headers rely on transitive includes of libc++; the standard
+// headers they use are included up front so that they compile with libstdc++.
+#define ENABLE_TEST
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <iterator>
+#include <limits>
+#include <optional>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <variant>
+#include <vector>
+
+#include <benchmark.hpp>
+#include <solver_ref.hpp>
+#include <solver_ref_naive.hpp>
+
+namespace gravity {
+using D = double;
+using CellD = Cell<D, D, D, D, D, D>;
+using TreeD = Octree<CellD, Vector<D>>;
+using SolverD = SolverRef<D, D, D, D, D, D>;
+using NaiveD = SolverRefNaive<D, D, D, D, D, D>;
+using ProblemD = Problem<D, D, D, D, D, D>;
+using BenchD = Benchmark<D, D, D, D, D, D>;
+using Lists = std::vector<std::vector<std::size_t>>;
+
+struct FMMTest {
+    static void link(const TreeD &tree, Lists &ap, Lists &ni, Lists &nb) {
+        const auto &cells = tree.cells();
+        ap.assign(cells.size(), {});
+        ni.assign(cells.size(), {});
+        nb.assign(cells.size(), {});
+        SolverD::calc_link(cells.front(), cells, ap, ni, nb, std::size_t{0});
+    }
+    static void fmm(const std::vector<Vector<D>> &x, D G, D m, D eps, std::vector<Vector<D>> &a, unsigned p, TreeD &tree, const Lists &ap,
+                    const Lists &ni) {
+        SolverD::fmm(x, G, m, eps, a, p, tree, ap, ni);
+    }
+    static std::complex<D> M(const CellD &c, unsigned p, unsigned k, int m) { return c._multipole.empty() ? std::complex<D>{} : c.multipole(p, k, m); }
+    static std::complex<D> L(const CellD &c, unsigned p, unsigned k, int m) { return c._local.empty() ? std::complex<D>{} : c.local(p, k, m); }
+    static std::vector<CellD> &cells(TreeD &t) { return t._cells; }
+    static bool has_local(const CellD &c) { return !c._local.empty(); }
+    static void divide(TreeD &t, std::size_t idx, std::size_t thr) { t.subdivide(idx, thr); }
+    // the pieces of SolverRef::fmm (solver_ref.hpp:180-232), split for timing
+    static void p2m(CellD &c, unsigned p, const std::vector<Vector<D>> &x, D m) { gravity::p2m(c, p, x, m); }
+    static void m2m(CellD &c, unsigned p, const std::vector<CellD> &cells) { gravity::m2m(c, p, cells); }
+    static void m2l(CellD &c, unsigned p, const std::vector<CellD> &cells, const std::vector<std::size_t> &ap) { gravity::m2l(c, p, cells, ap); }
+    static void l2l(CellD &c, unsigned p, const std::vector<CellD> &cells) { gravity::l2l(c, p, cells); }
+    static void l2p(const CellD &c, unsigned p, const std::vector<Vector<D>> &x, std::vector<Vector<D>> &a) { gravity::l2p(c, p, x, a); }
+    static void p2p(const CellD &c, const std::vector<Vector<D>> &x, std::vector<Vector<D>> &a, const std::vector<CellD> &cells,
+                    const std::vector<std::size_t> &ni, D G, D m, D eps) {
+        gravity::p2p(c, x, a, cells, ni, G, m, eps);
+    }
+};
+struct SolverRefTest {
+    static void rk4(std::vector<Vector<D>> &x, std::vector<Vector<D>> &v, D G, D m, D eps, D dt, std::size_t s, unsigned p) {
+        SolverD::runge_kutta(x, v, G, m, eps, dt, s, p);
+    }
+};
+struct SolverRefNaiveTest {
+    static void rk4(std::vector<Vector<D>> &x, std::vector<Vector<D>> &v, D G, D m, D eps, D dt) { NaiveD::runge_kutta(x, v, G, m, eps, dt); }
+    static Vector<D> acc(const std::vector<Vector<D>> &x, D G, D m, D eps, std::size_t i) { return NaiveD::acceleration(x, G, m, eps, i); }
+};
+struct BenchmarkTest {
+    static std::tuple<D, D> fit(const std::vector<D> &e, const std::vector<D> &t) { return BenchD::curve_fitting(e, t); }
+};
+}  // namespace gravity
+
+namespace {
+using namespace gravity;
+thread_local std::string g_err;
+
+std::vector<Vector<D>> to_vec(const double *x, int64_t n) {
+    std::vector<Vector<D>> v;
+    v.reserve(static_cast<size_t>(n));
+    for (int64_t i = 0; i < n; ++i) v.emplace_back(x[3 * i], x[3 * i + 1], x[3 * i + 2]);
+    return v;
+}
+void from_vec(const std::vector<Vector<D>> &v, double *x) {
+    for (size_t i = 0; i < v.size(); ++i) {
+        x[3 * i] = v[i].x();
+        x[3 * i + 1] = v[i].y();
+        x[3 * i + 2] = v[i].z();
+    }
+}
+template <class F>
+int guarded(F &&f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+}  // namespace
+
+struct ref_tree {
+    std::vector<Vector<D>> x;
+    TreeD *tree = nullptr;
+    Lists ap, ni, nb;
+    ~ref_tree() { delete tree; }
+};
+
+extern "C" {
+
+const char *ref_last_error(void) { return g_err.c_str(); }
+
+int ref_tree_build(const double *x, int64_t n, int64_t s, ref_tree **out) {
+    return guarded([&] {
+        auto *t = new ref_tree;
+        t->x = to_vec(x, n);
+        try {
+            t->tree = new TreeD{t->x, static_cast<std::size_t>(s)};
+        } catch (...) {
+            delete t;
+            throw;
+        }
+        *out = t;
+    });
+}
+
+void ref_tree_free(ref_tree *t) { delete t; }
+
+int64_t ref_tree_ncells(const ref_tree *t) { return static_cast<int64_t>(t->tree->cells().size()); }
+
+void ref_tree_export(const ref_tree *t, int64_t *parent, int32_t *level, double *bmin, double *bmax, int64_t *child_base, int64_t *pt_off,
+                     int64_t *pts) {
+    const auto &cells = t->tree->cells();
+    int64_t off = 0;
+    for (size_t c = 0; c < cells.size(); ++c) {
+        const auto &cell = cells[c];
+        parent[c] = cell.parent() ? static_cast<int64_t>(cell.parent().value()) : -1;
+        int32_t lev = 0;
+        for (auto q = cell.parent(); q; q = cells[q.value()].parent()) ++lev;
+        level[c] = lev;
+        bmin[3 * c] = cell.min().x();
+        bmin[3 * c + 1] = cell.min().y();
+        bmin[3 * c + 2] = cell.min().z();
+        bmax[3 * c] = cell.max().x();
+        bmax[3 * c + 1] = cell.max().y();
+        bmax[3 * c + 2] = cell.max().z();
+        pt_off[c] = off;
+        if (cell.is_leaf()) {
+            child_base[c] = -1;
+            for (const auto i : cell.points()) pts[off++] = static_cast<int64_t>(i);
+        } else {
+            child_base[c] = static_cast<int64_t>(*cell.cells().begin());
+        }
+    }
+    pt_off[cells.size()] = off;
+}
+
+int ref_tree_subdivide(ref_tree *t, int64_t idx, int64_t threshold) {
+    return guarded([&] {
+        FMMTest::divide(*t->tree, static_cast<std::size_t>(idx), static_cast<std::size_t>(threshold));
+        t->ap.clear();
+    });
+}
+
+int ref_tree_lists(ref_tree *t, int64_t *ap_total, int64_t *ni_total, int64_t *nb_total) {
+    return guarded([&] {
+        FMMTest::link(*t->tree, t->ap, t->ni, t->nb);
+        int64_t a = 0, b = 0, c = 0;
+        for (size_t k = 0; k < t->ap.size(); ++k) {
+            a += static_cast<int64_t>(t->ap[k].size());
+            b += static_cast<int64_t>(t->ni[k].size());
+            c += static_cast<int64_t>(t->nb[k].size());
+        }
+        *ap_total = a;
+        *ni_total = b;
+        *nb_total = c;
+    });
+}
+
+void ref_tree_lists_export(const ref_tree *t, int64_t *ap_off, int64_t *ap, int64_t *ni_off, int64_t *ni, int64_t *nb_off, int64_t *nb) {
+    int64_t a = 0, b = 0, c = 0;
+    for (size_t k = 0; k < t->ap.size(); ++k) {
+        ap_off[k] = a;
+        ni_off[k] = b;
+        nb_off[k] = c;
+        for (const auto v : t->ap[k]) ap[a++] = static_cast<int64_t>(v);
+        for (const auto v : t->ni[k]) ni[b++] = static_cast<int64_t>(v);
+        for (const auto v : t->nb[k]) nb[c++] = static_cast<int64_t>(v);
+    }
+    ap_off[t->ap.size()] = a;
+    ni_off[t->ap.size()] = b;
+    nb_off[t->ap.size()] = c;
+}
+
+// SolverRef::acceleration (solver_ref.hpp:164-177) on a built tree; M and L
+// exported per cell as (p+1)(2p+1) complex numbers [cell][n][p+m].
+int ref_tree_fmm(ref_tree *t, double m, double G, double eps, uint32_t p, double *a, double *M, double *L) {
+    return guarded([&] {
+        if (t->ap.empty()) FMMTest::link(*t->tree, t->ap, t->ni, t->nb);
+        std::vector<Vector<D>> acc(t->x.size(), Vector<D>{});
+        FMMTest::fmm(t->x, G, m, eps, acc, p, *t->tree, t->ap, t->ni);
+        from_vec(acc, a);
+        const auto &cells = t->tree->cells();
+        const size_t stride = (p + 1) * (2 * p + 1);
+        for (size_t c = 0; c < cells.size(); ++c)
+            for (unsigned k = 0; k <= p; ++k)
+                for (int mm = -static_cast<int>(p); mm <= static_cast<int>(p); ++mm) {
+                    const size_t o = c * stride + k * (2 * p + 1) + static_cast<size_t>(static_cast<int>(p) + mm);
+                    const auto mv = std::abs(mm) <= static_cast<int>(k) ? FMMTest::M(cells[c], p, k, mm) : std::complex<D>{};
+                    const auto lv = std::abs(mm) <= static_cast<int>(k) ? FMMTest::L(cells[c], p, k, mm) : std::complex<D>{};
+                    if (M) {
+                        M[2 * o] = mv.real();
+                        M[2 * o + 1] = mv.imag();
+                    }
+                    if (L) {
+                        L[2 * o] = lv.real();
+                        L[2 * o + 1] = lv.imag();
+                    }
+                }
+    });
+}
+
+int ref_rk4_fmm(double *x, double *v, int64_t n, double m, double G, double eps, double dt, int64_t s, uint32_t p) {
+    return guarded([&] {
+        auto xv = to_vec(x, n), vv = to_vec(v, n);
+        SolverRefTest::rk4(xv, vv, G, m, eps, dt, static_cast<std::size_t>(s), p);
+        from_vec(xv, x);
+        from_vec(vv, v);
+    });
+}
+
+int ref_rk4_direct(double *x, double *v, int64_t n, double m, double G, double eps, double dt) {
+    return guarded([&] {
+        auto xv = to_vec(x, n), vv = to_vec(v, n);
+        SolverRefNaiveTest::rk4(xv, vv, G, m, eps, dt);
+        from_vec(xv, x);
+        from_vec(vv, v);
+    });
+}
+
+void ref_direct_accel(const double *x, int64_t n, double m, double G, double eps, double *a) {
+    const auto xv = to_vec(x, n);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+        const auto ai = SolverRefNaiveTest::acc(xv, G, m, eps, static_cast<std::size_t>(i));
+        a[3 * i] = ai.x();
+        a[3 * i + 1] = ai.y();
+        a[3 * i + 2] = ai.z();
+    }
+}
+
+void ref_P(uint32_t nmax, double x, double *out) {
+    const P<D> t{nmax, nmax, x};
+    for (unsigned n = 0; n <= nmax; ++n)
+        for (unsigned m = 0; m <= nmax; ++m) out[n * (nmax + 1) + m] = m <= n ? t(n, m) : 0.0;
+}
+
+void ref_DP(uint32_t nmax, double theta, double *out) {
+    const DP<D> t{nmax, nmax, theta};
+    for (unsigned n = 0; n <= nmax; ++n)
+        for (unsigned m = 0; m <= nmax; ++m) out[n * (nmax + 1) + m] = m <= n ? t(n, m) : 0.0;
+}
+
+void ref_Y(uint32_t kmax, double theta, double phi, double *out) {
+    const Y<D> y{kmax, kmax, theta, phi};
+    for (unsigned k = 0; k <= kmax; ++k)
+        for (int m = -static_cast<int>(kmax); m <= static_cast<int>(kmax); ++m) {
+            const size_t o = k * (2 * kmax + 1) + static_cast<size_t>(static_cast<int>(kmax) + m);
+            const auto v = std::abs(m) <= static_cast<int>(k) ? y(k, m) : std::complex<D>{};
+            out[2 * o] = v.real();
+            out[2 * o + 1] = v.imag();
+        }
+}
+
+double ref_A(uint32_t n, int32_t m) { return A_<D, D, D, D, D, D>(n, m); }
+
+double ref_problem_initialize(int64_t n, uint64_t seed, double eps, double *x, double *v) {
+    const auto opt = eps < 0 ? std::optional<D>{} : std::optional<D>{eps};
+    const ProblemD prob{static_cast<std::size_t>(n), 10, opt, 1, 1, 1, 4, seed};
+    std::vector<Vector<D>> xv, vv;
+    prob.initialize(xv, vv);
+    from_vec(xv, x);
+    from_vec(vv, v);
+    return prob.epsilon();
+}
+
+double ref_problem_delta_t(double eps) { return ProblemD{1, 10, std::optional<D>{eps}}.delta_t(); }
+
+void ref_curve_fit(const double *e_t, const double *t, int64_t k, double *alpha, double *c) {
+    const std::vector<D> e(e_t, e_t + k), tt(t, t + k);
+    const auto [a, cc] = BenchmarkTest::fit(e, tt);
+    *alpha = a;
+    *c = cc;
+}
+
+// CPU-baseline timing of the reference FMM on a bounded sample of one force
+// evaluation.  Phases timed in full: tree build (Octree ctor) and calc_link.
+// Phases timed on a sample: P2M (all leaves), M2M, then M2L and P2P on the
+// first `m2l_cells` / `p2p_leaves` cells of a deterministic stride over the
+// cells; L2L/L2P on a sample likewise.  Outputs seconds per phase and the
+// exact full counts needed to scale each sampled phase up (written into
+// counts[]: [m2l_pairs_total, m2l_pairs_sampled, p2p_leaves_total,
+// p2p_pairs_total, p2p_pairs_sampled, l2_cells_total, l2_cells_sampled]).
+int ref_time_sample(const double *x, int64_t n, int64_t s, uint32_t p, double m, double G, double eps, int64_t m2l_cells, int64_t p2p_leaves,
+                    double *secs, double *counts) {
+    return guarded([&] {
+        using clk = std::chrono::steady_clock;
+        auto el = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); };
+        const auto xv = to_vec(x, n);
+        auto t0 = clk::now();
+        TreeD tree{xv, static_cast<std::size_t>(s)};
+        auto t1 = clk::now();
+        Lists ap, ni, nb;
+        FMMTest::link(tree, ap, ni, nb);
+        auto t2 = clk::now();
+        auto &cells = FMMTest::cells(tree);
+        std::vector<int> level(cells.size(), 0);
+        for (size_t c = 1; c < cells.size(); ++c) level[c] = level[cells[c].parent().value()] + 1;
+        // upward pass in full, deepest level first (equivalent order)
+        const int maxlev = *std::max_element(level.begin(), level.end());
+        for (int lev = maxlev; lev >= 2; --lev)
+            for (size_t c = 0; c < cells.size(); ++c)
+                if (level[c] == lev) {
+                    if (cells[c].is_leaf())
+                        FMMTest::p2m(cells[c], p, xv, m);
+                    else
+                        FMMTest::m2m(cells[c], p, cells);
+                }
+        auto t3 = clk::now();
+        // M2L sample: every k-th target cell with index > 8
+        double m2l_total = 0, m2l_sampled = 0;
+        std::vector<size_t> targets;
+        for (size_t c = 9; c < cells.size(); ++c) {
+            m2l_total += static_cast<double>(ap[c].size());
+            if (!ap[c].empty()) targets.push_back(c);
+        }
+        const size_t mstride = std::max<size_t>(1, targets.size() / static_cast<size_t>(std::max<int64_t>(1, m2l_cells)));
+        auto t4 = clk::now();
+        for (size_t k = 0; k < targets.size() && static_cast<int64_t>(k / mstride) < m2l_cells; k += mstride) {
+            FMMTest::m2l(cells[targets[k]], p, cells, ap[targets[k]]);
+            m2l_sampled += static_cast<double>(ap[targets[k]].size());
+        }
+        auto t5 = clk::now();
+        // P2P sample over leaves
+        std::vector<size_t> leaves;
+        for (size_t c = 0; c < cells.size(); ++c)
+            if (cells[c].is_leaf() && !cells[c].points().empty()) leaves.push_back(c);
+        auto subtree_count = [&](auto &&self, size_t c) -> double {
+            if (cells[c].is_leaf()) return static_cast<double>(cells[c].points().size());
+            double r = 0;
+            for (const auto ch : cells[c].cells()) r += self(self, ch);
+            return r;
+        };
+        double p2p_total = 0, p2p_sampled = 0;
+        std::vector<double> leaf_pairs(leaves.size());
+        for (size_t k = 0; k < leaves.size(); ++k) {
+            double src = 0;
+            for (const auto q : ni[leaves[k]]) src += subtree_count(subtree_count, q);
+            leaf_pairs[k] = src * static_cast<double>(cells[leaves[k]].points().size());
+            p2p_total += leaf_pairs[k];
+        }
+        std::vector<Vector<D>> acc(xv.size(), Vector<D>{});
+        const size_t pstride = std::max<size_t>(1, leaves.size() / static_cast<size_t>(std::max<int64_t>(1, p2p_leaves)));
+        auto t6 = clk::now();
+        for (size_t k = 0; k < leaves.size() && static_cast<int64_t>(k / pstride) < p2p_leaves; k += pstride) {
+            FMMTest::p2p(cells[leaves[k]], xv, acc, cells, ni[leaves[k]], G, m, eps);
+            p2p_sampled += leaf_pairs[k];
+        }
+        auto t7 = clk::now();
+        // downward pass sample: cells whose M2L ran (their L exists) at level >= 3 with a parent that has L
+        double l2_total = 0, l2_sampled = 0;
+        for (size_t c = 0; c < cells.size(); ++c)
+            if (level[c] >= 2) l2_total += 1;
+        auto t8 = clk::now();
+        for (size_t k = 0; k < targets.size() && static_cast<int64_t>(k / mstride) < m2l_cells; k += mstride) {
+            const size_t c = targets[k];
+            if (level[c] >= 3 && FMMTest::has_local(cells[cells[c].parent().value()])) FMMTest::l2l(cells[c], p, cells);
+            if (cells[c].is_leaf()) FMMTest::l2p(cells[c], p, xv, acc);
+            l2_sampled += 1;
+        }
+        auto t9 = clk::now();
+        secs[0] = el(t0, t1);  // tree
+        secs[1] = el(t1, t2);  // calc_link
+        secs[2] = el(t2, t3);  // P2M + M2M in full
+        secs[3] = el(t4, t5);  // M2L sample
+        secs[4] = el(t6, t7);  // P2P sample
+        secs[5] = el(t8, t9);  // L2L + L2P sample
+        counts[0] = m2l_total;
+        counts[1] = m2l_sampled;
+        counts[2] = static_cast<double>(leaves.size());
+        counts[3] = p2p_total;
+        counts[4] = p2p_sampled;
+        counts[5] = l2_total;
+        counts[6] = l2_sampled;
+    });
+}
+
+}  // extern "C"
