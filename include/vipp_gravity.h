@@ -1,0 +1,116 @@
+/*
+ * vipp_gravity.h - C ABI of the MI355X gravity FMM library (libvipp_gravity.so).
+ *
+ * This is the drop-in boundary for the reference's gravity hot path.  The
+ * reference selects its solver through the alias gravity::Solver
+ * (gravity/include/solver.hpp:7-8); a solver exposes preprocess / solve /
+ * postprocess (gravity/include/solver_ref.hpp:11-27) over
+ * std::vector<Vector<X>> positions and velocities, whose element type is three
+ * contiguous coordinates (gravity/include/problem.hpp:143-146), so a
+ * std::vector<Vector<double>> is exactly a double[3N].  The header-only C++
+ * wrapper vipp-nbody-bench_amd/include/gravity/solver_gpu.hpp maps those three
+ * calls onto the functions below.
+ *
+ * Conventions: plain pointers and sizes; every function returns VIPP_GRAV_OK
+ * (0) or an error code, and vipp_grav_last_error() gives the message (errors
+ * never throw across the ABI).  Host pointers are read or written before the
+ * call returns; none is retained.  Calls on one context must not overlap.
+ */
+#ifndef VIPP_GRAVITY_H
+#define VIPP_GRAVITY_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VIPP_GRAV_OK 0
+#define VIPP_GRAV_E_INVALID 1   /* bad argument or configuration */
+#define VIPP_GRAV_E_INPUT 2     /* non-finite coordinates (the reference would recurse without bound) */
+#define VIPP_GRAV_E_DEPTH 3     /* more than floor(sqrt(8) s) points closer than the 21-level resolution */
+#define VIPP_GRAV_E_DEVICE 4    /* HIP runtime error */
+#define VIPP_GRAV_E_STATE 5     /* call out of order (e.g. step before upload) */
+
+/* Arithmetic of the force evaluation.  The RK4 state (x, v) is FP64 in every mode. */
+#define VIPP_GRAV_F64 0         /* everything FP64: parity mode against the reference's double FMM */
+#define VIPP_GRAV_F32 1         /* FP32 P2P, FP32 expansions and plain FP32 M2L */
+#define VIPP_GRAV_F32_SPLIT 2   /* FP32 P2P; M2L on MFMA with BF16 hi/lo split operands, FP32 accumulation */
+
+/* Flags */
+#define VIPP_GRAV_FLAG_REF_FAR_G 1u  /* omit G from the far field exactly as the reference does (fmm.hpp:234,370-372) */
+
+typedef struct vipp_grav_config {
+    uint64_t n;          /* number of particles: Problem::N() (problem.hpp:214) */
+    uint64_t s;          /* leaf parameter: Problem::s() (problem.hpp:232); split above floor(sqrt(8) s) points (octree.hpp:83) */
+    uint32_t p;          /* expansion order: Problem::p() (problem.hpp:229); 1 <= p <= 12 */
+    uint32_t precision;  /* VIPP_GRAV_F64 / VIPP_GRAV_F32 / VIPP_GRAV_F32_SPLIT */
+    double epsilon;      /* Plummer softening: Problem::epsilon() (problem.hpp:217) */
+    double G;            /* gravitational constant: Problem::get_G() (problem.hpp:220) */
+    double mass;         /* per-particle mass when no mass array is uploaded: Problem::m() (problem.hpp:223) */
+    int32_t device;      /* HIP device ordinal */
+    uint32_t flags;      /* VIPP_GRAV_FLAG_* */
+} vipp_grav_config;
+
+typedef struct vipp_grav_ctx vipp_grav_ctx;
+
+/* Context lifetime (SolverRef is stateless, solver_ref.hpp:8-9; the context
+ * owns device memory between preprocess and postprocess). */
+int vipp_grav_create(const vipp_grav_config *cfg, vipp_grav_ctx **out);
+int vipp_grav_destroy(vipp_grav_ctx *ctx);
+/* Message of the last error on ctx (or of the last failed create when ctx is NULL). */
+const char *vipp_grav_last_error(const vipp_grav_ctx *ctx);
+
+/* State transfer: SolverRef::preprocess / postprocess (solver_ref.hpp:11-14).
+ * x, v: 3n doubles each (AoS x,y,z); masses: n doubles or NULL (scalar cfg.mass). */
+int vipp_grav_upload(vipp_grav_ctx *ctx, const double *x, const double *v, const double *masses);
+int vipp_grav_download(vipp_grav_ctx *ctx, double *x, double *v);
+
+/* nsteps classic RK4 steps of size dt (SolverRef::solve -> runge_kutta,
+ * solver_ref.hpp:17-27,36-104); each step rebuilds tree and lists at every
+ * stage as SolverRef::acceleration does (solver_ref.hpp:164-177).  Returns after
+ * the device work has finished. */
+int vipp_grav_step(vipp_grav_ctx *ctx, double dt, uint32_t nsteps);
+
+/* One force evaluation at the current positions (SolverRef::acceleration,
+ * solver_ref.hpp:164-177): a = 3n doubles (may be NULL to keep it on device). */
+int vipp_grav_accelerations(vipp_grav_ctx *ctx, double *a);
+
+/* Total kinetic and potential energy in FP64 at the current state (no
+ * counterpart in the reference; potential = FMM far field + softened near field). */
+int vipp_grav_energy(vipp_grav_ctx *ctx, double *kinetic, double *potential);
+
+/* Tree and lists of the last force evaluation (or built now if none), in the
+ * reference's cell numbering (octree.hpp:166-191): cells as in Octree::cells(),
+ * lists as calc_link builds them (solver_ref.hpp:115-162). */
+int vipp_grav_tree_info(vipp_grav_ctx *ctx, int64_t *ncells, int64_t *nleaves, int32_t *depth);
+int vipp_grav_export_tree(vipp_grav_ctx *ctx, int64_t *parent, int32_t *level, double *bmin, double *bmax,
+                          int64_t *child_base, int64_t *pt_off, int64_t *pts);
+int vipp_grav_export_lists_sizes(vipp_grav_ctx *ctx, int64_t *ap_total, int64_t *ni_total, int64_t *nb_total);
+int vipp_grav_export_lists(vipp_grav_ctx *ctx, int64_t *ap_off, int64_t *ap, int64_t *ni_off, int64_t *ni,
+                           int64_t *nb_off, int64_t *nb);
+/* Multipole and local expansions of the last evaluation in the reference's
+ * layout (Cell::_multipole/_local, fmm.hpp:563-566): per cell (p+1)(2p+1)
+ * complex values (re, im) indexed [n][p+m], reference numbering, unscaled. */
+int vipp_grav_export_expansions(vipp_grav_ctx *ctx, double *M, double *L);
+
+/* Work counts of the last evaluation: [0] P2P pairs (i != j counted as the
+ * reference sums them, self pairs excluded), [1] M2L cell pairs,
+ * [2] cells, [3] leaves, [4] depth. */
+int vipp_grav_counts(vipp_grav_ctx *ctx, double *out, int32_t n);
+
+/* Per-phase device milliseconds accumulated since the last reset, measured with
+ * HIP events on the context's stream: [0] tree, [1] lists, [2] upward (P2M,
+ * M2M), [3] M2L, [4] P2P, [5] downward (L2L, L2P), [6] RK4 updates,
+ * [7] operators.  Pass NULL to reset.  Enable timing with vipp_grav_set_timing. */
+int vipp_grav_phase_times(vipp_grav_ctx *ctx, double *ms, int32_t n);
+int vipp_grav_set_timing(vipp_grav_ctx *ctx, int32_t enabled);
+
+/* Device pointers of the FP64 state (3n each) for callers that keep data
+ * resident on the GPU; valid until destroy. */
+int vipp_grav_device_state(vipp_grav_ctx *ctx, void **x, void **v);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

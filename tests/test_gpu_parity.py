@@ -1,0 +1,218 @@
+"""Parity of the HIP path (libvipp_gravity.so through its C ABI) with the
+reference: golden vectors written by the reference's own code, and the CPU
+oracle on the same seeded inputs.
+
+Tolerances: trees, cell numbering, leaf lists and interaction lists are
+compared bit for bit.  FP64 accelerations and expansions are compared with a
+relative tolerance of 1e-11 of the largest value (summation order and libm
+differ from the reference).  The FP32 path is compared with the FP64 oracle at
+the FP32 accuracy it can reach (stated per test)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TREES = ["plummer16k_s64", "plummer4k_s5", "uniform2k_s1", "uniform20k_s64", "clump3k_s2"]
+
+
+@pytest.fixture(scope="module")
+def vg():
+    import vipp_gravity
+    return vipp_gravity
+
+
+def gpu_tree(vg, x, s, p=4, precision="f64", eps=0.0):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    with vg.Gravity(len(x), s, p, precision, eps=eps) as g:
+        g.upload(x)
+        return g.tree(), g.lists()
+
+
+def assert_tree_equal(t, ref):
+    for f in ("parent", "level", "child_base", "pt_off", "pts"):
+        assert np.array_equal(np.asarray(getattr(t, f)), np.asarray(getattr(ref, f) if hasattr(ref, f) else ref[f])), f
+    bmin = ref.bmin if hasattr(ref, "bmin") else ref["bmin"]
+    bmax = ref.bmax if hasattr(ref, "bmax") else ref["bmax"]
+    assert np.array_equal(t.bmin, bmin) and np.array_equal(t.bmax, bmax)
+
+
+def assert_lists_equal(l, ref):
+    for f in ("ap_off", "ap", "ni_off", "ni", "nb_off", "nb"):
+        assert np.array_equal(getattr(l, f), getattr(ref, f) if hasattr(ref, f) else ref[f]), f
+
+
+# ------------------------------------------------------------- tree / lists --
+@pytest.mark.parametrize("name", TREES)
+def test_tree_lists_golden(vg, golden, name):
+    g = golden("tree_" + name)
+    t, l = gpu_tree(vg, g["x"], int(g["s"]))
+    assert_tree_equal(t, g)
+    assert_lists_equal(l, g)
+
+
+@pytest.mark.parametrize("x", [
+    [[1, 1, 1], [-1, -1, -1], [1, -1, 1]],            # test_octree.cpp:113-128
+    [[4, 4, 4], [-4, -4, -4], [2, 2, 2], [3, 3, 3]],  # test_octree.cpp:130-154
+    [[0, 0, 0]],                                      # test_octree.cpp:105-111
+])
+def test_tree_kats(vg, orc, x):
+    x = np.array(x, float)
+    t, l = gpu_tree(vg, x, 1)
+    rt, rl = orc.tree_and_lists(x, 1)
+    assert_tree_equal(t, rt)
+    assert_lists_equal(l, rl)
+
+
+def test_tree_ties_on_split_planes(vg, orc):
+    """Points exactly on the midpoints of every level go to the high side (octree.hpp:136,153-158)."""
+    g = np.arange(-8, 9, dtype=float)
+    x = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    x = np.unique(np.concatenate([x, x[::7] * 0.5, x[::5] * 0.25]), axis=0)  # distinct points
+    for s in (1, 3, 10):
+        t, l = gpu_tree(vg, x, s)
+        rt, rl = orc.tree_and_lists(x, s)
+        assert_tree_equal(t, rt)
+        assert_lists_equal(l, rl)
+
+
+def test_tree_uniform_1m_s64(vg, orc):
+    """The 2^20-particle uniform workload: codes, tree and lists bit-exact."""
+    x = np.random.default_rng(1).random((1 << 20, 3))
+    t, l = gpu_tree(vg, x, 64, precision="f32")
+    rt, rl = orc.tree_and_lists(x, 64)
+    assert_tree_equal(t, rt)
+    assert_lists_equal(l, rl)
+
+
+def test_tree_plummer_16k(vg, orc):
+    x, _, _ = orc.problem_initialize(16384, 0, 1e-3)
+    t, l = gpu_tree(vg, x, 64)
+    rt, rl = orc.tree_and_lists(x, 64)
+    assert_tree_equal(t, rt)
+    assert_lists_equal(l, rl)
+
+
+# ----------------------------------------------------------------- FP64 FMM --
+@pytest.mark.parametrize("p", [2, 4, 8])
+def test_fmm_f64_golden(vg, golden, p):
+    g = golden("fmm_plummer2k")
+    x = g["x"]
+    with vg.Gravity(len(x), int(g["s"]), p, "f64", eps=float(g["eps"])) as gr:
+        gr.upload(x)
+        a = gr.accelerations()
+        ref = g[f"a_p{p}"]
+        assert np.abs(a - ref).max() <= 1e-11 * np.abs(ref).max()
+        if f"M_p{p}" in g:
+            M, L = gr.expansions()
+            t = gr.tree()
+            live = np.array([(t.level[c] >= 2) and (t.child_base[c] >= 0 or t.pt_off[c + 1] > t.pt_off[c]) for c in range(t.ncells)])
+            Mr, Lr = g[f"M_p{p}"], g[f"L_p{p}"]
+            assert np.abs(M[live] - Mr[live]).max() <= 1e-12 * np.abs(Mr).max()
+            assert np.abs(L[live] - Lr[live]).max() <= 1e-11 * np.abs(Lr[live]).max()
+
+
+def test_fmm_f64_plummer_p12_vs_direct(vg, orc):
+    x, _, _ = orc.problem_initialize(16384, 2, 1e-3)
+    d = orc.direct(x, eps=1e-3)
+    with vg.Gravity(len(x), 64, 12, "f64", eps=1e-3) as gr:
+        gr.upload(x)
+        a = gr.accelerations()
+    err = np.linalg.norm(a - d, axis=1) / np.linalg.norm(d, axis=1)
+    assert np.median(err) < 2e-6 and np.percentile(err, 99) < 1e-4, (np.median(err), np.percentile(err, 99))
+
+
+def test_fmm_f64_vs_oracle_masses_and_G(vg, orc):
+    rng = np.random.default_rng(5)
+    x = rng.random((6000, 3))
+    m = rng.uniform(0.5, 1.5, len(x)) / len(x)
+    for far_g in (False, True):
+        with vg.Gravity(len(x), 10, 6, "f64", eps=1e-4, G=2.5, ref_far_g=not far_g) as gr:
+            gr.upload(x, masses=m)
+            a = gr.accelerations()
+        ref = orc.fmm(x, 10, 6, G=2.5, eps=1e-4, masses=m, far_g=far_g)
+        assert np.abs(a - ref).max() <= 1e-11 * np.abs(ref).max()
+
+
+def test_rk4_step_f64_golden(vg, golden):
+    g = golden("rk4_1k")
+    x, v = g["x"], g["v"]
+    with vg.Gravity(len(x), int(g["s"]), int(g["p"]), "f64", eps=float(g["eps"])) as gr:
+        gr.upload(x, v)
+        gr.step(float(g["dt"]), 1)
+        x1, v1 = gr.download()
+    assert np.abs(x1 - g["x_fmm"]).max() <= 1e-11 * np.abs(g["x_fmm"] - x).max()
+    assert np.abs(v1 - g["v_fmm"]).max() <= 1e-11 * np.abs(g["v_fmm"] - v).max()
+
+
+def test_rk4_f64_two_body_closed_form(vg):
+    """test_solver_ref_naive.cpp:47-79 through the FMM path (two points: one leaf, pure P2P)."""
+    x = np.array([[0, 0, 0], [10, 0, 0]], float)
+    eps = 1e-4
+    with vg.Gravity(2, 1, 4, "f64", eps=eps, mass=1.0) as gr:
+        gr.upload(x)
+        gr.step(1.0, 1)
+        x1, v1 = gr.download()
+    d = 10.0
+    k1 = d / (d * d + eps * eps) ** 1.5
+    l2 = k1 / 2
+    k3 = (d - l2) / ((d - l2) ** 2 + eps * eps) ** 1.5
+    k4 = (d - 2 * l2) / ((d - 2 * l2) ** 2 + eps * eps) ** 1.5
+    xd = (2 * l2 + 2 * l2 + k3) / 6
+    vd = (k1 + 2 * k1 + 2 * k3 + k4) / 6
+    np.testing.assert_allclose(x1, [[xd, 0, 0], [d - xd, 0, 0]], atol=1e-15)
+    np.testing.assert_allclose(v1, [[vd, 0, 0], [-vd, 0, 0]], atol=1e-15)
+
+
+# ----------------------------------------------------------------- FP32 FMM --
+@pytest.mark.parametrize("p", [4, 8])
+def test_fmm_f32_vs_oracle(vg, orc, p):
+    """FP32 P2P/M2L against the FP64 oracle of the same algorithm: median 1e-6, 99th percentile 1e-4."""
+    x, _, _ = orc.problem_initialize(20000, 4, 1e-3)
+    ref = orc.fmm(x, 64, p, eps=1e-3)
+    with vg.Gravity(len(x), 64, p, "f32", eps=1e-3) as gr:
+        gr.upload(x)
+        a = gr.accelerations()
+    err = np.linalg.norm(a - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    assert np.median(err) < 1e-6 and np.percentile(err, 99) < 1e-4, (np.median(err), np.percentile(err, 99))
+
+
+def test_energy_f64(vg, orc):
+    x, v, _ = orc.problem_initialize(5000, 6, 1e-3)
+    _, pot = orc.direct(x, eps=1e-3, potential=True)
+    m = 1.0 / len(x)
+    with vg.Gravity(len(x), 10, 10, "f64", eps=1e-3) as gr:
+        gr.upload(x, v)
+        ke, pe = gr.energy()
+    assert abs(ke - 0.5 * m * (v ** 2).sum()) <= 1e-14 * ke
+    pe_ref = -0.5 * m * pot.sum()
+    assert abs(pe - pe_ref) <= 1e-6 * abs(pe_ref), (pe, pe_ref)
+
+
+# -------------------------------------------------------------- error paths --
+def test_non_finite_input_rejected(vg):
+    x = np.random.default_rng(0).random((100, 3))
+    x[17, 1] = np.nan
+    with vg.Gravity(len(x), 1, 4, "f32") as gr:
+        gr.upload(x)
+        with pytest.raises(vg.GravityError) as e:
+            gr.accelerations()
+        assert e.value.code == vg.E_INPUT
+
+
+def test_coincident_points_depth_error(vg):
+    x = np.concatenate([np.zeros((10, 3)), [[1.0, 1.0, 1.0]]])
+    with vg.Gravity(len(x), 1, 4, "f32") as gr:  # threshold 2 < 10 coincident points
+        gr.upload(x)
+        with pytest.raises(vg.GravityError) as e:
+            gr.accelerations()
+        assert e.value.code == vg.E_DEPTH
+
+
+def test_deterministic(vg, orc):
+    x, _, _ = orc.problem_initialize(30000, 8, 1e-3)
+    outs = []
+    for _ in range(2):
+        with vg.Gravity(len(x), 64, 8, "f32", eps=1e-3) as gr:
+            gr.upload(x)
+            outs.append(gr.accelerations())
+    assert np.array_equal(outs[0], outs[1])

@@ -1,0 +1,132 @@
+// gravity.hpp - device state of one gravity FMM context and the host-side
+// entry points of each phase.  Cell numbering on the device is level-major:
+// the cells of level d occupy ids [lvl_off[d], lvl_off[d+1]) and the eight
+// children of the k-th internal cell of level d are the consecutive ids
+// lvl_off[d+1] + 8k .. +7, in the reference's child-offset order
+// (octree.hpp:164-176: offset bit 0/1/2 == 0 is the high x/y/z half).
+#pragma once
+
+#include "common.hpp"
+
+namespace vg {
+
+enum Phase { PH_TREE = 0, PH_LISTS, PH_UP, PH_M2L, PH_P2P, PH_DOWN, PH_RK4, PH_OPS, PH_COUNT };
+
+struct TreeDev {
+    // host-side shape of the last build
+    int depth = 0;                 // deepest level holding cells
+    int64_t ncells = 0;
+    int64_t nleaves = 0;           // non-empty leaves
+    std::vector<int64_t> lvl_off;  // depth + 2 entries
+    std::vector<int64_t> int_off;  // internal cells of level d: int_ids[int_off[d] .. int_off[d+1])
+    double box[6] = {0, 0, 0, 0, 0, 0};  // padded root box (min xyz, max xyz)
+
+    // per cell
+    DevBuf<uint32_t> start, count;   // particle range in sorted order
+    DevBuf<int32_t> child, parent;   // first child (-1 for a leaf), parent (-1 for the root)
+    DevBuf<uint8_t> octant;          // child offset within the parent (0..7)
+    DevBuf<int8_t> level;
+    DevBuf<double> bmin, bmax;       // 3 per cell: bounds exactly as the reference forms them
+    DevBuf<int32_t> grid;            // 3 per cell: integer position at its level (low side = 0)
+    DevBuf<int32_t> anc;             // (MAXD+1) per cell: ancestor id at every level <= own level
+    DevBuf<int32_t> int_ids;         // internal cells, level by level
+    DevBuf<int32_t> leaf_ids;        // non-empty leaves
+    DevBuf<uint8_t> flags;           // scratch for compaction
+
+    // interaction lists (reference order, device ids)
+    DevBuf<int32_t> nb, ni, ap;              // 27, 27, 189 slots per cell
+    DevBuf<int32_t> nb_cnt, ni_cnt, ap_cnt;  // per cell
+    DevBuf<int32_t> nbr27;                   // 27 per cell: internal neighbour at offset (dx,dy,dz) or -1
+
+    // reference numbering (built on demand for export)
+    DevBuf<int32_t> ref_id;
+    bool ref_valid = false;
+};
+
+struct Ctx {
+    // configuration
+    int64_t n = 0;
+    int64_t s = 0;
+    uint32_t thr = 0;  // floor(sqrt(8) s)
+    int p = 4;
+    int K = 25;        // (p+1)^2 real coefficients
+    int prec = 1;
+    double eps = 0, G = 1, mass = 0;
+    uint32_t flags = 0;
+    int device = 0;
+    bool has_masses = false;
+    bool uploaded = false;
+    bool evaluated = false;
+    std::string err;
+
+    hipStream_t stream = nullptr;
+    bool timing = false;
+    hipEvent_t ev[2 * PH_COUNT] = {};
+    double phase_ms[PH_COUNT] = {};
+
+    // FP64 state in the caller's particle order
+    DevBuf<double> x, v, m, x0, v0, xo, vo;
+    // force of the last evaluation in the caller's order (FP64 or FP32 by mode)
+    DevBuf<double> acc64;
+    DevBuf<float> acc32;
+    DevBuf<double> pot64;  // potential (sum m/r without G) when requested
+
+    // tree build scratch
+    DevBuf<double> aabb_part;   // per-block partial boxes
+    DevBuf<double> box;         // 6 doubles + flag
+    DevBuf<uint64_t> key[2];
+    DevBuf<uint32_t> val[2];
+    DevBuf<uint32_t> rs_hist;
+    DevBuf<uint32_t> scal;      // small device scalars (counts, error flags)
+    DevBuf<uint32_t> perm;      // sorted position -> caller index
+    // particles in sorted order
+    DevBuf<double> xs64;        // 4 per particle: x, y, z, m
+    DevBuf<float> xs32;         // 4 per particle: x - root centre (FP32), m
+    DevBuf<double> as64;        // 4 per particle: accel xyz, potential
+    DevBuf<float> as32;
+
+    TreeDev t;
+
+    // expansions (scaled, real-packed (p+1)^2 per cell)
+    DevBuf<double> M64, L64;
+    DevBuf<float> M32, L32;
+    // translation operators
+    DevBuf<double> harm;        // harmonic tables of the operator offsets
+    DevBuf<double> op64;        // M2M (8), L2L (8), M2L (343) blocks of K*K
+    DevBuf<float> op32;
+    DevBuf<double> consts;      // A_n^m and recurrence constants (device copy)
+
+    // ref-numbering scratch
+    DevBuf<uint32_t> slist;
+    DevBuf<uint64_t> rkey[2];
+    DevBuf<uint32_t> rval[2];
+
+    double counts[8] = {};
+};
+
+// phases (each launches on ctx.stream)
+void tree_build(Ctx &c);                 // AABB, codes, sort, cells, leaves, particle gather
+void lists_build(Ctx &c);                // NB / NI / AP lists and the 27-neighbour table
+void ref_numbering(Ctx &c);              // reference cell ids (for export only)
+void operators_build(Ctx &c);            // M2M / L2L / M2L operators for the current root box
+void fmm_upward(Ctx &c);                 // P2M + M2M
+void fmm_m2l(Ctx &c);                    // M2L (plain VALU or split MFMA by mode)
+void fmm_p2p(Ctx &c, bool potential);    // near field into the sorted accelerations
+void fmm_downward(Ctx &c, bool potential);  // L2L + L2P
+void scatter_accel(Ctx &c, bool potential);  // sorted -> caller order
+void rk4_init(Ctx &c);
+void rk4_stage(Ctx &c, int stage, double dt);
+void count_work(Ctx &c);
+double kinetic_energy(Ctx &c);
+double potential_energy(Ctx &c);
+
+void phase_begin(Ctx &c, Phase p);
+void phase_end(Ctx &c, Phase p);
+
+// one force evaluation at the current positions
+void evaluate(Ctx &c, bool potential);
+
+void radix_sort_pairs(hipStream_t st, uint64_t *&k, uint32_t *&v, uint64_t *&k_alt, uint32_t *&v_alt, int64_t n, int begin_bit,
+                      int end_bit, DevBuf<uint32_t> &hist);
+
+}  // namespace vg

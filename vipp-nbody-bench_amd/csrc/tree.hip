@@ -1,0 +1,547 @@
+// tree.hip - octree build and interaction lists on the GPU.
+//
+// The reference inserts points one by one into an adaptive octree
+// (octree.hpp:79-191).  Its result does not depend on the insertion order:
+//  * a point descends by octant o = [x<mid] + 2[y<mid] + 4[z<mid] with
+//    mid = (max+min)/2 in FP64 and ties on the high side (first-fit over the
+//    children 0..7 with closed bounds, octree.hpp:136,153-158,169-176);
+//  * a cell has children iff it holds more than floor(sqrt(8) s) points
+//    (octree.hpp:83,143).
+// So every point gets a 63-bit path code (3 bits per level, 21 levels) by a
+// per-point FP64 midpoint descent, the codes are radix sorted (stable, so equal
+// codes keep the caller's order), and each level of cells is cut out of the
+// sorted codes with binary searches.  Leaves are then sorted by caller index,
+// which is the order the reference's leaf vectors have.
+#include "gravity.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace vg {
+namespace {
+
+constexpr double PADDING = 1e-5;  // Problem::OCTREE_WIDTH_PADDING (problem.hpp:167), (double)1E-5L == 1e-5
+
+__global__ __launch_bounds__(256) void k_aabb(const double *__restrict__ x, int64_t n, double *__restrict__ part) {
+    __shared__ double s[6][256];
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    double bad = 0;
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * 256) {
+        for (int a = 0; a < 3; ++a) {
+            const double v = x[3 * i + a];
+            if (!isfinite(v)) bad = 1;
+            lo[a] = fmin(lo[a], v);
+            hi[a] = fmax(hi[a], v);
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        s[a][threadIdx.x] = lo[a];
+        s[3 + a][threadIdx.x] = hi[a];
+    }
+    __shared__ double sb[256];
+    sb[threadIdx.x] = bad;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < static_cast<unsigned>(w)) {
+            for (int a = 0; a < 3; ++a) {
+                s[a][threadIdx.x] = fmin(s[a][threadIdx.x], s[a][threadIdx.x + w]);
+                s[3 + a][threadIdx.x] = fmax(s[3 + a][threadIdx.x], s[3 + a][threadIdx.x + w]);
+            }
+            sb[threadIdx.x] = fmax(sb[threadIdx.x], sb[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) part[blockIdx.x * 7 + threadIdx.x] = s[threadIdx.x][0];
+    if (threadIdx.x == 6) part[blockIdx.x * 7 + 6] = sb[0];
+}
+
+// final reduction; box = padded root bounds (fmm.hpp:403-405), box[6] = non-finite flag
+__global__ void k_aabb_final(const double *__restrict__ part, int nb, double *__restrict__ box) {
+    if (threadIdx.x != 0) return;
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY}, bad = 0;
+    for (int b = 0; b < nb; ++b) {
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = fmin(lo[a], part[7 * b + a]);
+            hi[a] = fmax(hi[a], part[7 * b + 3 + a]);
+        }
+        bad = fmax(bad, part[7 * b + 6]);
+    }
+    for (int a = 0; a < 3; ++a) {
+        box[a] = lo[a] - PADDING;
+        box[3 + a] = hi[a] + PADDING;
+    }
+    box[6] = bad;
+}
+
+// 63-bit path code: level-1 octant in bits 62..60, level-21 octant in bits 2..0.
+__global__ __launch_bounds__(256) void k_codes(const double *__restrict__ x, int64_t n, const double *__restrict__ box, uint64_t *__restrict__ key,
+                                               uint32_t *__restrict__ val) {
+    const int64_t i = blockIdx.x * 256ll + threadIdx.x;
+    if (i >= n) return;
+    double lo[3] = {box[0], box[1], box[2]}, hi[3] = {box[3], box[4], box[5]};
+    const double p[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
+    uint64_t code = 0;
+    for (int l = 0; l < MAXD; ++l) {
+        unsigned o = 0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const double mid = (hi[a] + lo[a]) / 2;  // octree.hpp:169
+            const bool low = p[a] < mid;
+            o |= static_cast<unsigned>(low) << a;
+            if (low)
+                hi[a] = mid;
+            else
+                lo[a] = mid;
+        }
+        code = (code << 3) | o;
+    }
+    key[i] = code;
+    val[i] = static_cast<uint32_t>(i);
+}
+
+__global__ void k_root(const double *__restrict__ box, int64_t n, uint32_t thr, uint32_t *start, uint32_t *count, int32_t *child, int32_t *parent,
+                       uint8_t *octant, int8_t *level, double *bmin, double *bmax, int32_t *grid, int32_t *anc, int32_t *int_ids, uint32_t *nint) {
+    if (threadIdx.x != 0) return;
+    start[0] = 0;
+    count[0] = static_cast<uint32_t>(n);
+    child[0] = -1;
+    parent[0] = -1;
+    octant[0] = 0;
+    level[0] = 0;
+    for (int a = 0; a < 3; ++a) {
+        // an empty input keeps the zero box of the reference (fmm.hpp:404-405)
+        bmin[a] = n ? box[a] : 0.0;
+        bmax[a] = n ? box[3 + a] : 0.0;
+        grid[a] = 0;
+    }
+    for (int l = 0; l <= MAXD; ++l) anc[l] = l == 0 ? 0 : -1;
+    const bool internal = static_cast<uint64_t>(n) > thr;
+    if (internal) int_ids[0] = 0;
+    *nint = internal ? 1u : 0u;
+}
+
+__device__ inline uint32_t digit_lower_bound(const uint64_t *__restrict__ key, uint32_t lo, uint32_t hi, int shift, unsigned q) {
+    // first position in [lo, hi) whose octal digit at `shift` is >= q (digits are sorted there)
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (((key[mid] >> shift) & 7u) < q)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Children of the internal cells of level d (octree.hpp:166-191).
+__global__ __launch_bounds__(256) void k_expand(int d, const int32_t *__restrict__ ids, int nint, int64_t child_off, const uint64_t *__restrict__ key,
+                                                uint32_t thr, uint32_t *start, uint32_t *count, int32_t *child, int32_t *parent, uint8_t *octant,
+                                                int8_t *level, double *bmin, double *bmax, int32_t *grid, int32_t *anc, uint8_t *flag) {
+    const int64_t t = blockIdx.x * 256ll + threadIdx.x;
+    if (t >= 8ll * nint) return;
+    const int k = static_cast<int>(t >> 3);
+    const unsigned q = static_cast<unsigned>(t & 7);
+    const int32_t P = ids[k];
+    const uint32_t s = start[P], e = s + count[P];
+    const int shift = 3 * (MAXD - 1 - d);
+    const uint32_t lo = digit_lower_bound(key, s, e, shift, q);
+    const uint32_t hi = q == 7 ? e : digit_lower_bound(key, lo, e, shift, q + 1);
+    const int64_t c = child_off + 8ll * k + q;
+    start[c] = lo;
+    count[c] = hi - lo;
+    child[c] = -1;
+    parent[c] = P;
+    octant[c] = static_cast<uint8_t>(q);
+    level[c] = static_cast<int8_t>(d + 1);
+    for (int a = 0; a < 3; ++a) {
+        const double mn = bmin[3 * P + a], mx = bmax[3 * P + a];
+        const double mid = (mx + mn) / 2;
+        const bool low = (q >> a) & 1u;
+        bmin[3 * c + a] = low ? mn : mid;
+        bmax[3 * c + a] = low ? mid : mx;
+        grid[3 * c + a] = 2 * grid[3 * P + a] + (low ? 0 : 1);
+    }
+    for (int l = 0; l <= MAXD; ++l) anc[c * (MAXD + 1) + l] = l <= d ? anc[static_cast<int64_t>(P) * (MAXD + 1) + l] : (l == d + 1 ? static_cast<int32_t>(c) : -1);
+    flag[8ll * k + q] = (hi - lo) > thr ? 1 : 0;
+    if (q == 0) child[P] = static_cast<int32_t>(child_off + 8ll * k);
+}
+
+// Ordered compaction in one block: out[j] = base + i for the j-th set flag.
+__global__ __launch_bounds__(1024) void k_compact(const uint8_t *__restrict__ flag, int64_t nflag, int64_t base, int32_t *__restrict__ out,
+                                                  uint32_t *__restrict__ nout) {
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    const unsigned lane = threadIdx.x & 63;
+    for (int64_t off = 0; off < nflag; off += 1024) {
+        const int64_t i = off + threadIdx.x;
+        const uint32_t f = (i < nflag && flag[i]) ? 1u : 0u;
+        const uint64_t bal = __ballot(f);
+        const uint32_t before = __popcll(bal & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
+        if (lane == 0) wsum[threadIdx.x >> 6] = __popcll(bal);
+        __syncthreads();
+        uint32_t wb = 0;
+        for (unsigned w = 0; w < (threadIdx.x >> 6); ++w) wb += wsum[w];
+        const uint32_t c0 = carry;
+        if (f) out[c0 + wb + before] = static_cast<int32_t>(base + i);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t tot = 0;
+            for (int w = 0; w < 16; ++w) tot += wsum[w];
+            carry = c0 + tot;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *nout = carry;
+}
+
+__global__ __launch_bounds__(256) void k_leaf_flags(const int32_t *__restrict__ child, const uint32_t *__restrict__ count, int64_t nc,
+                                                    uint8_t *__restrict__ flag) {
+    const int64_t c = blockIdx.x * 256ll + threadIdx.x;
+    if (c < nc) flag[c] = (child[c] < 0 && count[c] > 0) ? 1 : 0;
+}
+
+// Sort each leaf's caller indices ascending (a leaf holds at most thr points).
+__global__ __launch_bounds__(256) void k_leaf_sort(const int32_t *__restrict__ leaves, const uint32_t *__restrict__ start, const uint32_t *__restrict__ count,
+                                                   const uint32_t *__restrict__ in, uint32_t *__restrict__ out) {
+    extern __shared__ uint32_t sv[];
+    const int32_t c = leaves[blockIdx.x];
+    const uint32_t s = start[c], n = count[c];
+    for (uint32_t i = threadIdx.x; i < n; i += 256) sv[i] = in[s + i];
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += 256) {
+        const uint32_t v = sv[i];
+        uint32_t r = 0;
+        for (uint32_t j = 0; j < n; ++j) r += sv[j] < v;
+        out[s + r] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gather(const uint32_t *__restrict__ perm, int64_t n, const double *__restrict__ x, const double *__restrict__ m,
+                                                double mass, const double *__restrict__ box, double *__restrict__ xs64, float *__restrict__ xs32) {
+    const int64_t k = blockIdx.x * 256ll + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t i = perm[k];
+    const double px = x[3ll * i], py = x[3ll * i + 1], pz = x[3ll * i + 2];
+    const double mi = m ? m[i] : mass;
+    xs64[4 * k] = px;
+    xs64[4 * k + 1] = py;
+    xs64[4 * k + 2] = pz;
+    xs64[4 * k + 3] = mi;
+    const double cx = (box[0] + box[3]) / 2, cy = (box[1] + box[4]) / 2, cz = (box[2] + box[5]) / 2;
+    xs32[4 * k] = static_cast<float>(px - cx);
+    xs32[4 * k + 1] = static_cast<float>(py - cy);
+    xs32[4 * k + 2] = static_cast<float>(pz - cz);
+    xs32[4 * k + 3] = static_cast<float>(mi);
+}
+
+// ---------------------------------------------------------------------------
+// Interaction lists, level by level (calc_link, solver_ref.hpp:115-162)
+// ---------------------------------------------------------------------------
+__global__ void k_lists_root(const int32_t *__restrict__ child, int32_t *nb, int32_t *ni, int32_t *nb_cnt, int32_t *ni_cnt, int32_t *ap_cnt, int32_t *nbr27) {
+    if (threadIdx.x != 0) return;
+    for (int q = 0; q < 27; ++q) nbr27[q] = -1;
+    if (child[0] < 0) {  // solver_ref.hpp:149-150
+        ni[0] = 0;
+        ni_cnt[0] = 1;
+        nb_cnt[0] = 0;
+    } else {  // solver_ref.hpp:151-152
+        nb[0] = 0;
+        nb_cnt[0] = 1;
+        ni_cnt[0] = 0;
+        nbr27[13] = 0;
+    }
+    ap_cnt[0] = 0;
+}
+
+__global__ __launch_bounds__(128) void k_lists_level(int64_t c0, int64_t c1, const int32_t *__restrict__ parent, const int32_t *__restrict__ child,
+                                                     const int32_t *__restrict__ grid, int32_t *__restrict__ nb, int32_t *__restrict__ ni,
+                                                     int32_t *__restrict__ ap, int32_t *__restrict__ nb_cnt, int32_t *__restrict__ ni_cnt,
+                                                     int32_t *__restrict__ ap_cnt, int32_t *__restrict__ nbr27, uint32_t *__restrict__ overflow) {
+    const int64_t c = c0 + blockIdx.x * 128ll + threadIdx.x;
+    if (c >= c1) return;
+    const int32_t P = parent[c];
+    const int gx = grid[3 * c], gy = grid[3 * c + 1], gz = grid[3 * c + 2];
+    const bool leaf = child[c] < 0;
+    int nbn = 0, nin = 0, apn = 0;
+    int32_t *nbc = nb + 27 * c, *nic = ni + 27 * c, *apc = ap + 189 * c;
+    int32_t *n27 = nbr27 + 27 * c;
+    for (int q = 0; q < 27; ++q) n27[q] = -1;
+    const int np = nb_cnt[P];
+    for (int e = 0; e < np; ++e) {  // solver_ref.hpp:129-147
+        const int32_t pn = nb[27 * static_cast<int64_t>(P) + e];
+        const int32_t base = child[pn];
+        for (int q = 0; q < 8; ++q) {
+            const int32_t cc = base + q;
+            const int dx = grid[3 * cc] - gx, dy = grid[3 * cc + 1] - gy, dz = grid[3 * cc + 2] - gz;
+            const bool adj = dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1;  // fmm.hpp:475-493
+            if (adj) {
+                if (!leaf && child[cc] >= 0) {
+                    if (nbn < 27) nbc[nbn] = cc;
+                    ++nbn;
+                    n27[(dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)] = cc;
+                } else {
+                    if (nin < 27) nic[nin] = cc;
+                    ++nin;
+                }
+            } else {
+                if (apn < 189) apc[apn] = cc;
+                ++apn;
+            }
+        }
+    }
+    if (nbn > 27 || nin > 27 || apn > 189) atomicOr(overflow, 1u);
+    nb_cnt[c] = min(nbn, 27);
+    ni_cnt[c] = min(nin, 27);
+    ap_cnt[c] = min(apn, 189);
+}
+
+// ---------------------------------------------------------------------------
+// Reference numbering (for export): children of the internal cell that
+// subdivided r-th are 1 + 8r .. 8 + 8r (octree.hpp:170-177).  A cell
+// subdivides when its (thr+1)-th point in caller order arrives; cells that
+// subdivide on the same arrival do so parent first.  So internal cells are
+// ranked by (t = (thr+1)-th smallest caller index of the cell, depth).
+// S(cell) = its thr+1 smallest caller indices, merged bottom-up.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_smallest(const int32_t *__restrict__ ids, int n_ids, const int32_t *__restrict__ child, const uint32_t *__restrict__ start,
+                                                  const uint32_t *__restrict__ count, const uint32_t *__restrict__ perm, const int32_t *__restrict__ int_index,
+                                                  uint32_t *__restrict__ slist, uint32_t keep, uint64_t *__restrict__ rkey, uint32_t *__restrict__ rval,
+                                                  const int8_t *__restrict__ level, int pow2) {
+    extern __shared__ uint32_t buf[];
+    const int32_t P = ids[blockIdx.x];
+    const int32_t base = child[P];
+    // gather candidates
+    __shared__ uint32_t off[9];
+    if (threadIdx.x == 0) {
+        uint32_t o = 0;
+        for (int q = 0; q < 8; ++q) {
+            off[q] = o;
+            const int32_t c = base + q;
+            o += child[c] < 0 ? count[c] : keep;
+        }
+        off[8] = o;
+    }
+    __syncthreads();
+    for (int q = 0; q < 8; ++q) {
+        const int32_t c = base + q;
+        const uint32_t n = off[q + 1] - off[q];
+        const uint32_t *src = child[c] < 0 ? perm + start[c] : slist + static_cast<size_t>(int_index[c]) * keep;
+        for (uint32_t i = threadIdx.x; i < n; i += 256) buf[off[q] + i] = src[i];
+    }
+    for (uint32_t i = off[8] + threadIdx.x; i < static_cast<uint32_t>(pow2); i += 256) buf[i] = 0xFFFFFFFFu;
+    __syncthreads();
+    // bitonic sort of pow2 entries
+    for (int k = 2; k <= pow2; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < pow2; i += 256) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const uint32_t a = buf[i], b = buf[l];
+                    const bool up = (i & k) == 0;
+                    if ((a > b) == up) {
+                        buf[i] = b;
+                        buf[l] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    const int32_t me = int_index[P];
+    for (uint32_t i = threadIdx.x; i < keep; i += 256) slist[static_cast<size_t>(me) * keep + i] = buf[i];
+    if (threadIdx.x == 0) {
+        rkey[me] = (static_cast<uint64_t>(buf[keep - 1]) << 5) | static_cast<uint64_t>(level[P]);
+        rval[me] = static_cast<uint32_t>(me);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_int_index(const int32_t *__restrict__ int_ids, int64_t nint, int32_t *__restrict__ int_index) {
+    const int64_t k = blockIdx.x * 256ll + threadIdx.x;
+    if (k < nint) int_index[int_ids[k]] = static_cast<int32_t>(k);
+}
+
+__global__ __launch_bounds__(256) void k_rank_of(const uint32_t *__restrict__ sorted_val, int64_t nint, int32_t *__restrict__ rank) {
+    const int64_t r = blockIdx.x * 256ll + threadIdx.x;
+    if (r < nint) rank[sorted_val[r]] = static_cast<int32_t>(r);
+}
+
+__global__ __launch_bounds__(256) void k_ref_ids(int64_t nc, const int32_t *__restrict__ parent, const uint8_t *__restrict__ octant,
+                                                 const int32_t *__restrict__ int_index, const int32_t *__restrict__ rank, int32_t *__restrict__ ref_id) {
+    const int64_t c = blockIdx.x * 256ll + threadIdx.x;
+    if (c >= nc) return;
+    const int32_t P = parent[c];
+    ref_id[c] = P < 0 ? 0 : 1 + 8 * rank[int_index[P]] + octant[c];
+}
+
+}  // namespace
+
+void tree_build(Ctx &c) {
+    hipStream_t st = c.stream;
+    TreeDev &t = c.t;
+    const int64_t n = c.n;
+    // 1. root box
+    const int nb = static_cast<int>(std::min<int64_t>(1024, std::max<int64_t>(1, (n + 255) / 256)));
+    c.aabb_part.ensure(7 * static_cast<size_t>(nb));
+    c.box.ensure(8);
+    c.scal.ensure(16);
+    k_aabb<<<nb, 256, 0, st>>>(c.x, n, c.aabb_part);
+    k_aabb_final<<<1, 64, 0, st>>>(c.aabb_part, nb, c.box);
+    VG_LAUNCH_CHECK();
+    // 2. path codes and stable sort
+    for (int b = 0; b < 2; ++b) {
+        c.key[b].ensure(std::max<int64_t>(1, n));
+        c.val[b].ensure(std::max<int64_t>(1, n));
+    }
+    uint64_t *k0 = c.key[0].p, *k1 = c.key[1].p;
+    uint32_t *v0 = c.val[0].p, *v1 = c.val[1].p;
+    if (n) {
+        k_codes<<<ceil_div(n, 256), 256, 0, st>>>(c.x, n, c.box, k0, v0);
+        VG_LAUNCH_CHECK();
+        radix_sort_pairs(st, k0, v0, k1, v1, n, 0, 3 * MAXD, c.rs_hist);
+    }
+    // 3. cells, level by level
+    t.lvl_off.assign(1, 0);
+    t.int_off.assign(1, 0);
+    int64_t cap = 1024;
+    auto ensure_cells = [&](int64_t ncell, bool keep) {
+        t.start.ensure(ncell, keep, st);
+        t.count.ensure(ncell, keep, st);
+        t.child.ensure(ncell, keep, st);
+        t.parent.ensure(ncell, keep, st);
+        t.octant.ensure(ncell, keep, st);
+        t.level.ensure(ncell, keep, st);
+        t.bmin.ensure(3 * ncell, keep, st);
+        t.bmax.ensure(3 * ncell, keep, st);
+        t.grid.ensure(3 * ncell, keep, st);
+        t.anc.ensure((MAXD + 1) * ncell, keep, st);
+    };
+    ensure_cells(cap, false);
+    t.int_ids.ensure(1024, false, st);
+    k_root<<<1, 64, 0, st>>>(c.box, n, c.thr, t.start, t.count, t.child, t.parent, t.octant, t.level, t.bmin, t.bmax, t.grid, t.anc, t.int_ids,
+                            c.scal.p);
+    VG_LAUNCH_CHECK();
+    uint32_t hbuf[2];
+    double hbox[8];
+    VG_HIP(hipMemcpyAsync(hbox, c.box, 7 * sizeof(double), hipMemcpyDeviceToHost, st));
+    VG_HIP(hipMemcpyAsync(hbuf, c.scal.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    VG_HIP(hipStreamSynchronize(st));
+    if (hbox[6] != 0) throw Error(2, "non-finite particle coordinates");
+    for (int a = 0; a < 6; ++a) t.box[a] = hbox[a];
+    int64_t nint = hbuf[0];
+    int64_t ncells = 1;
+    t.lvl_off.push_back(1);
+    t.int_off.push_back(nint);
+    int d = 0;
+    while (nint > 0) {
+        if (d >= MAXD)
+            throw Error(3, "octree deeper than 21 levels: more than floor(sqrt(8) s) points closer than the root box / 2^21");
+        const int64_t nnew = 8 * nint;
+        if (ncells + nnew > cap) {
+            while (cap < ncells + nnew) cap *= 2;
+            ensure_cells(cap, true);
+        }
+        c.t.flags.ensure(nnew, false, st);
+        t.int_ids.ensure(t.int_off.back() + nnew, true, st);
+        k_expand<<<ceil_div(nnew, 256), 256, 0, st>>>(d, t.int_ids.p + t.int_off[d], static_cast<int>(nint), ncells, k0, c.thr, t.start, t.count,
+                                                       t.child, t.parent, t.octant, t.level, t.bmin, t.bmax, t.grid, t.anc, t.flags);
+        k_compact<<<1, 1024, 0, st>>>(t.flags, nnew, ncells, t.int_ids.p + t.int_off[d + 1], c.scal.p);
+        VG_LAUNCH_CHECK();
+        VG_HIP(hipMemcpyAsync(hbuf, c.scal.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        VG_HIP(hipStreamSynchronize(st));
+        ncells += nnew;
+        t.lvl_off.push_back(ncells);
+        nint = hbuf[0];
+        t.int_off.push_back(t.int_off.back() + nint);
+        ++d;
+    }
+    t.depth = static_cast<int>(t.lvl_off.size()) - 2;
+    t.ncells = ncells;
+    // 4. non-empty leaves, leaf order by caller index, particles in sorted order
+    t.flags.ensure(ncells, false, st);
+    t.leaf_ids.ensure(ncells, false, st);
+    k_leaf_flags<<<ceil_div(ncells, 256), 256, 0, st>>>(t.child, t.count, ncells, t.flags);
+    k_compact<<<1, 1024, 0, st>>>(t.flags, ncells, 0, t.leaf_ids, c.scal.p);
+    VG_LAUNCH_CHECK();
+    VG_HIP(hipMemcpyAsync(hbuf, c.scal.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    VG_HIP(hipStreamSynchronize(st));
+    t.nleaves = hbuf[0];
+    c.perm.ensure(std::max<int64_t>(1, n));
+    if (t.nleaves) {
+        k_leaf_sort<<<static_cast<unsigned>(t.nleaves), 256, c.thr * sizeof(uint32_t), st>>>(t.leaf_ids, t.start, t.count, v0, c.perm);
+        VG_LAUNCH_CHECK();
+    }
+    c.xs64.ensure(4 * std::max<int64_t>(1, n));
+    c.xs32.ensure(4 * std::max<int64_t>(1, n));
+    if (n) {
+        k_gather<<<ceil_div(n, 256), 256, 0, st>>>(c.perm, n, c.x, c.has_masses ? c.m.p : nullptr, c.mass, c.box, c.xs64, c.xs32);
+        VG_LAUNCH_CHECK();
+    }
+    t.ref_valid = false;
+}
+
+void lists_build(Ctx &c) {
+    hipStream_t st = c.stream;
+    TreeDev &t = c.t;
+    const int64_t nc = t.ncells;
+    t.nb.ensure(27 * nc);
+    t.ni.ensure(27 * nc);
+    t.ap.ensure(189 * nc);
+    t.nb_cnt.ensure(nc);
+    t.ni_cnt.ensure(nc);
+    t.ap_cnt.ensure(nc);
+    t.nbr27.ensure(27 * nc);
+    VG_HIP(hipMemsetAsync(c.scal.p + 4, 0, sizeof(uint32_t), st));
+    k_lists_root<<<1, 64, 0, st>>>(t.child, t.nb, t.ni, t.nb_cnt, t.ni_cnt, t.ap_cnt, t.nbr27);
+    for (int d = 1; d <= t.depth; ++d) {
+        const int64_t c0 = t.lvl_off[d], c1 = t.lvl_off[d + 1];
+        k_lists_level<<<ceil_div(c1 - c0, 128), 128, 0, st>>>(c0, c1, t.parent, t.child, t.grid, t.nb, t.ni, t.ap, t.nb_cnt, t.ni_cnt, t.ap_cnt,
+                                                             t.nbr27, c.scal.p + 4);
+    }
+    VG_LAUNCH_CHECK();
+}
+
+void ref_numbering(Ctx &c) {
+    hipStream_t st = c.stream;
+    TreeDev &t = c.t;
+    const int64_t nc = t.ncells;
+    const int64_t nint = t.int_off.back();
+    t.ref_id.ensure(std::max<int64_t>(1, nc));
+    if (nint == 0) {
+        k_ref_ids<<<ceil_div(nc, 256), 256, 0, st>>>(nc, t.parent, t.octant, nullptr, nullptr, t.ref_id);
+        VG_LAUNCH_CHECK();
+        t.ref_valid = true;
+        return;
+    }
+    const uint32_t keep = c.thr + 1;
+    int pow2 = 1;
+    while (static_cast<uint32_t>(pow2) < 8 * keep) pow2 <<= 1;
+    if (pow2 * sizeof(uint32_t) > 64 * 1024) throw Error(1, "export of the reference numbering supports floor(sqrt(8) s) < 2048");
+    DevBuf<int32_t> int_index, rank;
+    int_index.ensure(nc);
+    rank.ensure(nint);
+    c.slist.ensure(static_cast<size_t>(nint) * keep);
+    for (int b = 0; b < 2; ++b) {
+        c.rkey[b].ensure(nint);
+        c.rval[b].ensure(nint);
+    }
+    k_int_index<<<ceil_div(nint, 256), 256, 0, st>>>(t.int_ids, nint, int_index);
+    for (int d = t.depth; d >= 0; --d) {
+        const int64_t a = t.int_off[d], b = t.int_off[d + 1];
+        if (b > a)
+            k_smallest<<<static_cast<unsigned>(b - a), 256, pow2 * sizeof(uint32_t), st>>>(t.int_ids.p + a, static_cast<int>(b - a), t.child, t.start,
+                                                                                            t.count, c.perm, int_index, c.slist, keep, c.rkey[0], c.rval[0],
+                                                                                            t.level, pow2);
+    }
+    VG_LAUNCH_CHECK();
+    uint64_t *k0 = c.rkey[0].p, *k1 = c.rkey[1].p;
+    uint32_t *v0 = c.rval[0].p, *v1 = c.rval[1].p;
+    radix_sort_pairs(st, k0, v0, k1, v1, nint, 0, 64, c.rs_hist);
+    k_rank_of<<<ceil_div(nint, 256), 256, 0, st>>>(v0, nint, rank);
+    k_ref_ids<<<ceil_div(nc, 256), 256, 0, st>>>(nc, t.parent, t.octant, int_index, rank, t.ref_id);
+    VG_LAUNCH_CHECK();
+    VG_HIP(hipStreamSynchronize(st));
+    t.ref_valid = true;
+}
+
+}  // namespace vg
