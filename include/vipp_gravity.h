@@ -95,8 +95,10 @@ int vipp_grav_export_lists(vipp_grav_ctx *ctx, int64_t *ap_off, int64_t *ap, int
 int vipp_grav_export_expansions(vipp_grav_ctx *ctx, double *M, double *L);
 
 /* Work counts of the last evaluation: [0] P2P pairs (i != j counted as the
- * reference sums them, self pairs excluded), [1] M2L cell pairs,
- * [2] cells, [3] leaves, [4] depth. */
+ * reference sums them, self pairs excluded), [1] M2L cell pairs (every AP
+ * entry of every cell at level >= 2, as the reference evaluates them),
+ * [2] cells, [3] non-empty leaves, [4] depth, [5] M2L pairs whose source and
+ * target both hold particles (the pairs that carry work). */
 int vipp_grav_counts(vipp_grav_ctx *ctx, double *out, int32_t n);
 
 /* Per-phase device milliseconds accumulated since the last reset, measured with

@@ -59,6 +59,10 @@ struct FMMTest {
     static std::complex<D> L(const CellD &c, unsigned p, unsigned k, int m) { return c._local.empty() ? std::complex<D>{} : c.local(p, k, m); }
     static std::vector<CellD> &cells(TreeD &t) { return t._cells; }
     static bool has_local(const CellD &c) { return !c._local.empty(); }
+    static void presize(CellD &c, unsigned p) {
+        c._multipole.assign(p + 1, std::vector<std::complex<D>>(2 * p + 1));
+        c._local.assign(p + 1, std::vector<std::complex<D>>(2 * p + 1));
+    }
     static void divide(TreeD &t, std::size_t idx, std::size_t thr) { t.subdivide(idx, thr); }
     // the pieces of SolverRef::fmm (solver_ref.hpp:180-232), split for timing
     static void p2m(CellD &c, unsigned p, const std::vector<Vector<D>> &x, D m) { gravity::p2m(c, p, x, m); }
@@ -308,16 +312,17 @@ void ref_curve_fit(const double *e_t, const double *t, int64_t k, double *alpha,
     *c = cc;
 }
 
-// CPU-baseline timing of the reference FMM on a bounded sample of one force
-// evaluation.  Phases timed in full: tree build (Octree ctor) and calc_link.
-// Phases timed on a sample: P2M (all leaves), M2M, then M2L and P2P on the
-// first `m2l_cells` / `p2p_leaves` cells of a deterministic stride over the
-// cells; L2L/L2P on a sample likewise.  Outputs seconds per phase and the
-// exact full counts needed to scale each sampled phase up (written into
-// counts[]: [m2l_pairs_total, m2l_pairs_sampled, p2p_leaves_total,
-// p2p_pairs_total, p2p_pairs_sampled, l2_cells_total, l2_cells_sampled]).
-int ref_time_sample(const double *x, int64_t n, int64_t s, uint32_t p, double m, double G, double eps, int64_t m2l_cells, int64_t p2p_leaves,
-                    double *secs, double *counts) {
+// CPU-baseline timing of the reference FMM (SolverRef::acceleration,
+// solver_ref.hpp:164-232) on a bounded sample of one force evaluation.
+// The tree (Octree ctor) and calc_link run in full.  Every other phase runs
+// on an evenly strided sample of its work items and is scaled up by the exact
+// ratio of work units (particles for P2M/L2P, cells for M2M/L2L, AP entries for
+// M2L, particle pairs for P2P).  Expansion storage is sized up front (zeros;
+// the operators' cost does not depend on the values).
+//   out[0..6]  extrapolated seconds: tree, link, P2M, M2M, M2L, P2P, L2L+L2P
+//   info[0..6] sampled fraction of each phase (1 for the full phases)
+int ref_time_sample(const double *x, int64_t n, int64_t s, uint32_t p, double m, double G, double eps, int64_t per_phase, double *out,
+                    double *info) {
     return guarded([&] {
         using clk = std::chrono::steady_clock;
         auto el = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); };
@@ -328,85 +333,76 @@ int ref_time_sample(const double *x, int64_t n, int64_t s, uint32_t p, double m,
         Lists ap, ni, nb;
         FMMTest::link(tree, ap, ni, nb);
         auto t2 = clk::now();
+        out[0] = el(t0, t1);
+        out[1] = el(t1, t2);
+        info[0] = info[1] = 1;
         auto &cells = FMMTest::cells(tree);
-        std::vector<int> level(cells.size(), 0);
-        for (size_t c = 1; c < cells.size(); ++c) level[c] = level[cells[c].parent().value()] + 1;
-        // upward pass in full, deepest level first (equivalent order)
-        const int maxlev = *std::max_element(level.begin(), level.end());
-        for (int lev = maxlev; lev >= 2; --lev)
-            for (size_t c = 0; c < cells.size(); ++c)
-                if (level[c] == lev) {
-                    if (cells[c].is_leaf())
-                        FMMTest::p2m(cells[c], p, xv, m);
-                    else
-                        FMMTest::m2m(cells[c], p, cells);
-                }
-        auto t3 = clk::now();
-        // M2L sample: every k-th target cell with index > 8
-        double m2l_total = 0, m2l_sampled = 0;
-        std::vector<size_t> targets;
-        for (size_t c = 9; c < cells.size(); ++c) {
-            m2l_total += static_cast<double>(ap[c].size());
-            if (!ap[c].empty()) targets.push_back(c);
-        }
-        const size_t mstride = std::max<size_t>(1, targets.size() / static_cast<size_t>(std::max<int64_t>(1, m2l_cells)));
-        auto t4 = clk::now();
-        for (size_t k = 0; k < targets.size() && static_cast<int64_t>(k / mstride) < m2l_cells; k += mstride) {
-            FMMTest::m2l(cells[targets[k]], p, cells, ap[targets[k]]);
-            m2l_sampled += static_cast<double>(ap[targets[k]].size());
-        }
-        auto t5 = clk::now();
-        // P2P sample over leaves
-        std::vector<size_t> leaves;
-        for (size_t c = 0; c < cells.size(); ++c)
-            if (cells[c].is_leaf() && !cells[c].points().empty()) leaves.push_back(c);
+        const size_t nc = cells.size();
+        std::vector<int> level(nc, 0);
+        for (size_t c = 1; c < nc; ++c) level[c] = level[cells[c].parent().value()] + 1;
+        for (auto &c : cells) FMMTest::presize(c, p);
+        std::vector<Vector<D>> acc(xv.size(), Vector<D>{});
+        // select every k-th of `items` (weights w) up to per_phase items, run f, scale by weight ratio
+        auto sampled = [&](const std::vector<size_t> &items, const std::vector<double> &w, auto &&f, double &secs, double &frac) {
+            double tot = 0, done = 0;
+            for (double q : w) tot += q;
+            if (items.empty() || tot == 0) {
+                secs = 0;
+                frac = 1;
+                return;
+            }
+            const size_t stride = std::max<size_t>(1, items.size() / static_cast<size_t>(std::max<int64_t>(1, per_phase)));
+            auto a = clk::now();
+            for (size_t k = 0; k < items.size(); k += stride) {
+                f(items[k]);
+                done += w[k];
+            }
+            auto b = clk::now();
+            frac = done / tot;
+            secs = el(a, b) / frac;
+        };
+        std::vector<size_t> leaves2, inner2, targets, leavesall, l2cells;
+        std::vector<double> wl2, wi2, wt, wla, wl2c;
         auto subtree_count = [&](auto &&self, size_t c) -> double {
             if (cells[c].is_leaf()) return static_cast<double>(cells[c].points().size());
             double r = 0;
             for (const auto ch : cells[c].cells()) r += self(self, ch);
             return r;
         };
-        double p2p_total = 0, p2p_sampled = 0;
-        std::vector<double> leaf_pairs(leaves.size());
-        for (size_t k = 0; k < leaves.size(); ++k) {
-            double src = 0;
-            for (const auto q : ni[leaves[k]]) src += subtree_count(subtree_count, q);
-            leaf_pairs[k] = src * static_cast<double>(cells[leaves[k]].points().size());
-            p2p_total += leaf_pairs[k];
+        for (size_t c = 0; c < nc; ++c) {
+            const bool leaf = cells[c].is_leaf();
+            if (level[c] >= 2 && leaf && !cells[c].points().empty()) {
+                leaves2.push_back(c);
+                wl2.push_back(static_cast<double>(cells[c].points().size()));
+            }
+            if (level[c] >= 2 && !leaf) {
+                inner2.push_back(c);
+                wi2.push_back(1);
+            }
+            if (c > 8 && !ap[c].empty()) {
+                targets.push_back(c);
+                wt.push_back(static_cast<double>(ap[c].size()));
+            }
+            if (leaf && !cells[c].points().empty()) {
+                double src = 0;
+                for (const auto q : ni[c]) src += subtree_count(subtree_count, q);
+                leavesall.push_back(c);
+                wla.push_back(src * static_cast<double>(cells[c].points().size()));
+            }
+            if (level[c] >= 3) {
+                l2cells.push_back(c);
+                wl2c.push_back(1);
+            }
         }
-        std::vector<Vector<D>> acc(xv.size(), Vector<D>{});
-        const size_t pstride = std::max<size_t>(1, leaves.size() / static_cast<size_t>(std::max<int64_t>(1, p2p_leaves)));
-        auto t6 = clk::now();
-        for (size_t k = 0; k < leaves.size() && static_cast<int64_t>(k / pstride) < p2p_leaves; k += pstride) {
-            FMMTest::p2p(cells[leaves[k]], xv, acc, cells, ni[leaves[k]], G, m, eps);
-            p2p_sampled += leaf_pairs[k];
-        }
-        auto t7 = clk::now();
-        // downward pass sample: cells whose M2L ran (their L exists) at level >= 3 with a parent that has L
-        double l2_total = 0, l2_sampled = 0;
-        for (size_t c = 0; c < cells.size(); ++c)
-            if (level[c] >= 2) l2_total += 1;
-        auto t8 = clk::now();
-        for (size_t k = 0; k < targets.size() && static_cast<int64_t>(k / mstride) < m2l_cells; k += mstride) {
-            const size_t c = targets[k];
-            if (level[c] >= 3 && FMMTest::has_local(cells[cells[c].parent().value()])) FMMTest::l2l(cells[c], p, cells);
-            if (cells[c].is_leaf()) FMMTest::l2p(cells[c], p, xv, acc);
-            l2_sampled += 1;
-        }
-        auto t9 = clk::now();
-        secs[0] = el(t0, t1);  // tree
-        secs[1] = el(t1, t2);  // calc_link
-        secs[2] = el(t2, t3);  // P2M + M2M in full
-        secs[3] = el(t4, t5);  // M2L sample
-        secs[4] = el(t6, t7);  // P2P sample
-        secs[5] = el(t8, t9);  // L2L + L2P sample
-        counts[0] = m2l_total;
-        counts[1] = m2l_sampled;
-        counts[2] = static_cast<double>(leaves.size());
-        counts[3] = p2p_total;
-        counts[4] = p2p_sampled;
-        counts[5] = l2_total;
-        counts[6] = l2_sampled;
+        sampled(leaves2, wl2, [&](size_t c) { FMMTest::p2m(cells[c], p, xv, m); }, out[2], info[2]);
+        sampled(inner2, wi2, [&](size_t c) { FMMTest::m2m(cells[c], p, cells); }, out[3], info[3]);
+        sampled(targets, wt, [&](size_t c) { FMMTest::m2l(cells[c], p, cells, ap[c]); }, out[4], info[4]);
+        sampled(leavesall, wla, [&](size_t c) { FMMTest::p2p(cells[c], xv, acc, cells, ni[c], G, m, eps); }, out[5], info[5]);
+        double s1 = 0, f1 = 1, s2 = 0, f2 = 1;
+        sampled(l2cells, wl2c, [&](size_t c) { FMMTest::l2l(cells[c], p, cells); }, s1, f1);
+        sampled(leaves2, wl2, [&](size_t c) { FMMTest::l2p(cells[c], p, xv, acc); }, s2, f2);
+        out[6] = s1 + s2;
+        info[6] = std::min(f1, f2);
     });
 }
 

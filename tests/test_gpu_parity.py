@@ -176,6 +176,32 @@ def test_fmm_f32_vs_oracle(vg, orc, p):
     assert np.median(err) < 1e-6 and np.percentile(err, 99) < 1e-4, (np.median(err), np.percentile(err, 99))
 
 
+@pytest.mark.parametrize("p", [4, 8, 12])
+def test_fmm_split_vs_oracle(vg, orc, p):
+    """BF16 hi/lo split M2L on MFMA (FP32 elsewhere) against the FP64 oracle: median 2e-6, 99th percentile 1e-4."""
+    x, _, _ = orc.problem_initialize(20000, 4, 1e-3)
+    ref = orc.fmm(x, 64, p, eps=1e-3)
+    with vg.Gravity(len(x), 64, p, "split", eps=1e-3) as gr:
+        gr.upload(x)
+        a = gr.accelerations()
+    err = np.linalg.norm(a - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    assert np.median(err) < 2e-6 and np.percentile(err, 99) < 1e-4, (np.median(err), np.percentile(err, 99))
+
+
+def test_split_m2l_locals_match_plain(vg, orc):
+    """The local expansions from the MFMA path agree with the plain FP32 path to the split's 2^-16 precision."""
+    x, _, _ = orc.problem_initialize(30000, 5, 1e-3)
+    Ls = {}
+    for mode in ("f32", "split"):
+        with vg.Gravity(len(x), 64, 8, mode, eps=1e-3) as gr:
+            gr.upload(x)
+            gr.accelerations(copy=False)
+            Ls[mode] = gr.expansions()[1]
+    a, b = Ls["f32"], Ls["split"]
+    scale = np.abs(a).max(axis=(1, 2), keepdims=True) + 1e-300
+    assert (np.abs(a - b) / scale).max() < 1e-4
+
+
 def test_energy_f64(vg, orc):
     x, v, _ = orc.problem_initialize(5000, 6, 1e-3)
     _, pot = orc.direct(x, eps=1e-3, potential=True)

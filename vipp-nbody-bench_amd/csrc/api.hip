@@ -178,6 +178,7 @@ int vipp_grav_create(const vipp_grav_config *cfg, vipp_grav_ctx **out) {
             c.thr = static_cast<uint32_t>(thr);
             c.p = static_cast<int>(cfg->p);
             c.K = (c.p + 1) * (c.p + 1);
+            c.ldk = c.prec == 2 ? (c.K + 15) / 16 * 16 : c.K;
             c.prec = static_cast<int>(cfg->precision);
             c.eps = cfg->epsilon;
             c.G = cfg->G;
@@ -436,7 +437,7 @@ int vipp_grav_export_expansions(vipp_grav_ctx *ctx, double *M, double *L) {
         VG_HIP(hipSetDevice(c.device));
         ensure_tree(c);
         const vg::TreeDev &t = c.t;
-        const size_t nc = static_cast<size_t>(t.ncells), K = static_cast<size_t>(c.K);
+        const size_t nc = static_cast<size_t>(t.ncells), K = static_cast<size_t>(c.ldk);
         std::vector<double> Mh(nc * K), Lh(nc * K);
         if (c.prec == 0) {
             Mh = fetch(c, c.M64.p, nc * K);

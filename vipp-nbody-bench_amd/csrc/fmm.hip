@@ -209,7 +209,7 @@ __global__ __launch_bounds__(64) void k_ops(const double *__restrict__ harm, int
 template <class R>
 __global__ __launch_bounds__(64) void k_p2m(const int32_t *__restrict__ leaves, const int8_t *__restrict__ level, const uint32_t *__restrict__ start,
                                             const uint32_t *__restrict__ count, const double *__restrict__ bmin, const double *__restrict__ bmax,
-                                            const double *__restrict__ xs64, double wx0, int p, int K, R *__restrict__ M) {
+                                            const double *__restrict__ xs64, double wx0, int p, int K, int ldk, R *__restrict__ M) {
     constexpr int ROWS = 32;
     __shared__ R rows[ROWS * MAXK];
     const int32_t c = leaves[blockIdx.x];
@@ -263,19 +263,20 @@ __global__ __launch_bounds__(64) void k_p2m(const int32_t *__restrict__ leaves, 
         }
         __syncthreads();
     }
-    for (int q = lane, slot = 0; q < K; q += 64, ++slot) M[static_cast<size_t>(c) * K + q] = acc[slot];
+    for (int q = lane, slot = 0; q < K; q += 64, ++slot) M[static_cast<size_t>(c) * ldk + q] = acc[slot];
+    for (int q = K + lane; q < ldk; q += 64) M[static_cast<size_t>(c) * ldk + q] = R(0);
 }
 
 // M2M for the internal cells of one level (solver_ref.hpp:196-197).
 template <class R>
 __global__ __launch_bounds__(256) void k_m2m(const int32_t *__restrict__ ids, const int32_t *__restrict__ child, const uint32_t *__restrict__ count,
-                                             const R *__restrict__ op, int K, R *__restrict__ M) {
+                                             const R *__restrict__ op, int K, int ldk, R *__restrict__ M) {
     __shared__ R cm[8 * MAXK];
     const int32_t P = ids[blockIdx.x];
     const int32_t base = child[P];
     for (int i = threadIdx.x; i < 8 * K; i += 256) {
         const int q = i / K, col = i % K;
-        cm[i] = count[base + q] ? M[static_cast<size_t>(base + q) * K + col] : R(0);
+        cm[i] = count[base + q] ? M[static_cast<size_t>(base + q) * ldk + col] : R(0);
     }
     __syncthreads();
     for (int r = threadIdx.x; r < K; r += 256) {
@@ -289,18 +290,19 @@ __global__ __launch_bounds__(256) void k_m2m(const int32_t *__restrict__ ids, co
             const R *mv = cm + q * K;
             for (int col = 0; col < cend; ++col) sum += orow[col] * mv[col];
         }
-        M[static_cast<size_t>(P) * K + r] = sum;
+        M[static_cast<size_t>(P) * ldk + r] = sum;
     }
+    for (int q = K + threadIdx.x; q < ldk; q += 256) M[static_cast<size_t>(P) * ldk + q] = R(0);
 }
 
 // L2L from the internal cells of one level to their children (solver_ref.hpp:216-217).
 template <class R>
 __global__ __launch_bounds__(256) void k_l2l(const int32_t *__restrict__ ids, const int32_t *__restrict__ child, const uint32_t *__restrict__ count,
-                                             const R *__restrict__ op, int K, R *__restrict__ L) {
+                                             const R *__restrict__ op, int K, int ldk, R *__restrict__ L) {
     __shared__ R pl[MAXK];
     const int32_t P = ids[blockIdx.x];
     const int32_t base = child[P];
-    for (int i = threadIdx.x; i < K; i += 256) pl[i] = L[static_cast<size_t>(P) * K + i];
+    for (int i = threadIdx.x; i < K; i += 256) pl[i] = L[static_cast<size_t>(P) * ldk + i];
     __syncthreads();
     for (int i = threadIdx.x; i < 8 * K; i += 256) {
         const int q = i / K, r = i % K;
@@ -310,7 +312,7 @@ __global__ __launch_bounds__(256) void k_l2l(const int32_t *__restrict__ ids, co
         while ((j + 1) * (j + 1) <= r) ++j;
         R sum = 0;
         for (int col = j * j; col < K; ++col) sum += orow[col] * pl[col];  // parent degrees >= j
-        L[static_cast<size_t>(base + q) * K + r] += sum;
+        L[static_cast<size_t>(base + q) * ldk + r] += sum;
     }
 }
 
@@ -323,11 +325,12 @@ __global__ __launch_bounds__(256) void k_l2l(const int32_t *__restrict__ ids, co
 template <class R, int TB>
 __global__ __launch_bounds__(256) void k_m2l_plain(const int32_t *__restrict__ parents, int nparents, const int32_t *__restrict__ child,
                                                    const uint32_t *__restrict__ count, const int32_t *__restrict__ nbr27, const R *__restrict__ Mx,
-                                                   R *__restrict__ Lx, const R *__restrict__ op, int K) {
+                                                   R *__restrict__ Lx, const R *__restrict__ op, int K, int ldk) {
     constexpr int KC = 16;
     constexpr int CPT = TB / 8;  // targets per thread
-    __shared__ R sM[MAXK * TB];
-    __shared__ R sOp[KC * 192];
+    constexpr int SMS = TB + 1, SOS = 193;  // padded strides: no bank conflicts
+    __shared__ R sM[MAXK * SMS];
+    __shared__ R sOp[KC * SOS];
     __shared__ int32_t par[TB], tgt[TB], src[TB];
     __shared__ int any;
     const int qt = blockIdx.y;
@@ -371,23 +374,23 @@ __global__ __launch_bounds__(256) void k_m2l_plain(const int32_t *__restrict__ p
             for (int i = threadIdx.x; i < K * TB; i += 256) {
                 const int t = i / K, col = i % K;
                 const int32_t s = src[t];
-                sM[col * TB + t] = s >= 0 ? Mx[static_cast<size_t>(s) * K + col] : R(0);
+                sM[col * SMS + t] = s >= 0 ? Mx[static_cast<size_t>(s) * ldk + col] : R(0);
             }
             const R *O = op + static_cast<size_t>(16 + o) * K * K;
             for (int c0 = 0; c0 < K; c0 += KC) {
                 __syncthreads();
                 for (int i = threadIdx.x; i < KC * K; i += 256) {
                     const int r = i / KC, cc = i % KC;
-                    sOp[cc * 192 + r] = (c0 + cc < K) ? O[static_cast<size_t>(r) * K + c0 + cc] : R(0);
+                    sOp[cc * SOS + r] = (c0 + cc < K) ? O[static_cast<size_t>(r) * K + c0 + cc] : R(0);
                 }
                 __syncthreads();
                 const int ce = min(KC, K - c0);
                 for (int cc = 0; cc < ce; ++cc) {
                     R mv[CPT];
-                    for (int jj = 0; jj < CPT; ++jj) mv[jj] = sM[(c0 + cc) * TB + tc + 8 * jj];
+                    for (int jj = 0; jj < CPT; ++jj) mv[jj] = sM[(c0 + cc) * SMS + tc + 8 * jj];
 #pragma unroll
                     for (int i = 0; i < 6; ++i) {
-                        const R ov = sOp[cc * 192 + tr + 32 * i];
+                        const R ov = sOp[cc * SOS + tr + 32 * i];
 #pragma unroll
                         for (int jj = 0; jj < CPT; ++jj) acc[i][jj] += ov * mv[jj];
                     }
@@ -401,7 +404,7 @@ __global__ __launch_bounds__(256) void k_m2l_plain(const int32_t *__restrict__ p
         if (t < 0) continue;
         for (int i = 0; i < 6; ++i) {
             const int r = tr + 32 * i;
-            if (r < K) Lx[static_cast<size_t>(t) * K + r] = acc[i][jj];
+            if (r < K) Lx[static_cast<size_t>(t) * ldk + r] = acc[i][jj];
         }
     }
 }
@@ -413,12 +416,12 @@ __global__ __launch_bounds__(256) void k_m2l_plain(const int32_t *__restrict__ p
 template <class R>
 __global__ __launch_bounds__(64) void k_l2p(const int32_t *__restrict__ leaves, const int8_t *__restrict__ level, const uint32_t *__restrict__ start,
                                             const uint32_t *__restrict__ count, const double *__restrict__ bmin, const double *__restrict__ bmax,
-                                            const double *__restrict__ xs64, double wx0, int p, int K, const R *__restrict__ Lx, double gscale,
-                                            R *__restrict__ acc4) {
+                                            const double *__restrict__ xs64, double wx0, int p, int K, int ldk, const R *__restrict__ Lx,
+                                            double gscale, R *__restrict__ acc4) {
     __shared__ R sl[MAXK];
     const int32_t c = leaves[blockIdx.x];
     if (level[c] < 2) return;  // solver_ref.hpp:220-221
-    for (int i = threadIdx.x; i < K; i += 64) sl[i] = Lx[static_cast<size_t>(c) * K + i];
+    for (int i = threadIdx.x; i < K; i += 64) sl[i] = Lx[static_cast<size_t>(c) * ldk + i];
     __syncthreads();
     const uint32_t s = start[c], cnt = count[c];
     const double w = ldexp(wx0, -level[c]);
@@ -491,8 +494,8 @@ struct Bufs<double> {
     static double *op(Ctx &c) { return c.op64; }
     static double *acc(Ctx &c) { return c.as64; }
     static void ensure(Ctx &c, size_t nc) {
-        c.M64.ensure(nc * c.K);
-        c.L64.ensure(nc * c.K);
+        c.M64.ensure(nc * c.ldk + 16);
+        c.L64.ensure(nc * c.ldk + 16);
     }
 };
 template <>
@@ -502,8 +505,8 @@ struct Bufs<float> {
     static float *op(Ctx &c) { return c.op32; }
     static float *acc(Ctx &c) { return c.as32; }
     static void ensure(Ctx &c, size_t nc) {
-        c.M32.ensure(nc * c.K);
-        c.L32.ensure(nc * c.K);
+        c.M32.ensure(nc * c.ldk + 16);
+        c.L32.ensure(nc * c.ldk + 16);
     }
 };
 
@@ -514,10 +517,10 @@ void upward_t(Ctx &c) {
     const double wx0 = t.box[3] - t.box[0];
     if (t.nleaves)
         k_p2m<R><<<static_cast<unsigned>(t.nleaves), 64, 0, c.stream>>>(t.leaf_ids, t.level, t.start, t.count, t.bmin, t.bmax, c.xs64, wx0, c.p,
-                                                                       c.K, Bufs<R>::M(c));
+                                                                       c.K, c.ldk, Bufs<R>::M(c));
     for (int d = t.depth; d >= 2; --d) {
         const int64_t a = t.int_off[d], b = t.int_off[d + 1];
-        if (b > a) k_m2m<R><<<static_cast<unsigned>(b - a), 256, 0, c.stream>>>(t.int_ids.p + a, t.child, t.count, Bufs<R>::op(c), c.K, Bufs<R>::M(c));
+        if (b > a) k_m2m<R><<<static_cast<unsigned>(b - a), 256, 0, c.stream>>>(t.int_ids.p + a, t.child, t.count, Bufs<R>::op(c), c.K, c.ldk, Bufs<R>::M(c));
     }
     VG_LAUNCH_CHECK();
 }
@@ -532,7 +535,7 @@ void m2l_plain_t(Ctx &c) {
     constexpr int TB = sizeof(R) == 4 ? 32 : 16;
     dim3 grid(ceil_div(b - a, TB), 8);
     k_m2l_plain<R, TB><<<grid, 256, 0, c.stream>>>(t.int_ids.p + a, static_cast<int>(b - a), t.child, t.count, t.nbr27, Bufs<R>::M(c), Bufs<R>::L(c),
-                                                   Bufs<R>::op(c), c.K);
+                                                   Bufs<R>::op(c), c.K, c.ldk);
     VG_LAUNCH_CHECK();
 }
 
@@ -541,17 +544,19 @@ void downward_t(Ctx &c) {
     TreeDev &t = c.t;
     for (int d = 2; d < t.depth; ++d) {  // parents at level d >= 2 feed children at level >= 3
         const int64_t a = t.int_off[d], b = t.int_off[d + 1];
-        if (b > a) k_l2l<R><<<static_cast<unsigned>(b - a), 256, 0, c.stream>>>(t.int_ids.p + a, t.child, t.count, Bufs<R>::op(c), c.K, Bufs<R>::L(c));
+        if (b > a) k_l2l<R><<<static_cast<unsigned>(b - a), 256, 0, c.stream>>>(t.int_ids.p + a, t.child, t.count, Bufs<R>::op(c), c.K, c.ldk, Bufs<R>::L(c));
     }
     const double wx0 = t.box[3] - t.box[0];
     const double gscale = (c.flags & 1u) ? 1.0 : c.G;
     if (t.nleaves)
         k_l2p<R><<<static_cast<unsigned>(t.nleaves), 64, 0, c.stream>>>(t.leaf_ids, t.level, t.start, t.count, t.bmin, t.bmax, c.xs64, wx0, c.p, c.K,
-                                                                       Bufs<R>::L(c), gscale, Bufs<R>::acc(c));
+                                                                       c.ldk, Bufs<R>::L(c), gscale, Bufs<R>::acc(c));
     VG_LAUNCH_CHECK();
 }
 
 }  // namespace
+
+void m2l_split_operators(Ctx &c, int nvec);
 
 void fmm_constants_init() {
     static bool done = false;
@@ -624,6 +629,11 @@ void operators_build(Ctx &c) {
     if (c.prec == 0) {
         c.op64.ensure(static_cast<size_t>(nvec) * c.K * c.K);
         k_ops<double><<<grid, 64, 0, c.stream>>>(c.harm, c.p, c.K, c.op64);
+    } else if (c.prec == 2) {
+        c.op64.ensure(static_cast<size_t>(nvec) * c.K * c.K);
+        k_ops<double><<<grid, 64, 0, c.stream>>>(c.harm, c.p, c.K, c.op64);
+        c.op32.ensure(static_cast<size_t>(nvec) * c.K * c.K);
+        m2l_split_operators(c, nvec);
     } else {
         c.op32.ensure(static_cast<size_t>(nvec) * c.K * c.K);
         k_ops<float><<<grid, 64, 0, c.stream>>>(c.harm, c.p, c.K, c.op32);

@@ -50,6 +50,7 @@ struct Ctx {
     uint32_t thr = 0;  // floor(sqrt(8) s)
     int p = 4;
     int K = 25;        // (p+1)^2 real coefficients
+    int ldk = 25;      // row stride of the expansion arrays (K, or K rounded up to 16 for the MFMA path)
     int prec = 1;
     double eps = 0, G = 1, mass = 0;
     uint32_t flags = 0;
@@ -94,12 +95,19 @@ struct Ctx {
     DevBuf<double> harm;        // harmonic tables of the operator offsets
     DevBuf<double> op64;        // M2M (8), L2L (8), M2L (343) blocks of K*K
     DevBuf<float> op32;
+    DevBuf<uint16_t> opsplit;   // M2L operators as BF16 hi/lo MFMA fragments
     DevBuf<double> consts;      // A_n^m and recurrence constants (device copy)
 
     // ref-numbering scratch
     DevBuf<uint32_t> slist;
     DevBuf<uint64_t> rkey[2];
     DevBuf<uint32_t> rval[2];
+
+    // P2P work items
+    DevBuf<uint32_t> p2p_scal, p2p_nchunk, p2p_item_off;
+    DevBuf<int64_t> p2p_items;  // int2 (leaf index, chunk)
+    DevBuf<float> p2p_part32;
+    DevBuf<double> p2p_part64;
 
     double counts[8] = {};
 };

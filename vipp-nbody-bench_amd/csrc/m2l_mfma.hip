@@ -1,8 +1,267 @@
-// m2l_mfma.hip - M2L on the matrix cores (split BF16 operands); see fmm_m2l_split.
+// m2l_mfma.hip - M2L on the CDNA4 matrix cores with split BF16 operands.
+//
+// The M2L of one target is L~(t) = sum_o T_o M~(s(t, o)) over its (at most
+// 189) source offsets o (fmm.hpp:277-304 in the scaled real basis of
+// fmm.hip).  A block owns 32 NT targets of one child octant qt, so that every
+// target sees the same 189 offsets in the same order (offsets follow from
+// (parent-neighbour offset, source octant, qt) alone).  Per offset the block
+// forms T_o (K x K) times the targets' source multipoles (K x 32 NT) on
+// v_mfma_f32_32x32x16_bf16, one wavefront per 32 output rows, accumulating
+// in FP32 across all offsets.  Every block walks the offsets in the same
+// order, so the operators stream through L2 shared by the blocks of an XCD.
+//
+// Precision: every operand x is split as hi = bf16(x), lo = bf16(x - hi)
+// (16 significant bits together) and a product is formed as
+// hi*hi + hi*lo + lo*hi with FP32 accumulation; the dropped lo*lo term and the
+// split residuals are ~2^-18 of the product.  T_o is split from its FP64
+// value; the multipoles are FP32 and are split in registers.
+//
+// Operand layouts (checked on the device): for v_mfma_f32_32x32x16_bf16 lane l
+// holds A[l%32][8(l/32)+e] and B[8(l/32)+e][l%32], e = 0..7, and
+// C[8(r/4) + 4(l/32) + r%4][l%32], r = 0..15.  T_o is stored pre-swizzled in
+// that A order, so a wavefront loads a fragment with one 16-byte read per lane.
 #include "gravity.hpp"
 
 namespace vg {
+namespace {
 
-void fmm_m2l_split(Ctx &) { throw Error(1, "split-precision M2L is not built in this library"); }
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+
+__host__ __device__ constexpr int mt_of(int K) { return (K + 31) / 32; }
+__host__ __device__ constexpr int ks_of(int K) { return (K + 15) / 16; }
+
+// FP64 operators (row-major K x K per offset) -> FP32 copies (all blocks) and
+// BF16 hi/lo A fragments of the M2L blocks: frag[o][mt][h][ks][lane][e], so
+// that any run of consecutive m-tiles (hi and lo) is one contiguous range.
+__global__ __launch_bounds__(256) void k_split_ops(const double *__restrict__ op64, int nblk, int K, float *__restrict__ op32, __bf16 *__restrict__ frag,
+                                                   int MT, int NKS) {
+    const int64_t i = blockIdx.x * 256ll + threadIdx.x;
+    const int64_t kk = static_cast<int64_t>(K) * K;
+    if (i < nblk * kk) op32[i] = static_cast<float>(op64[i]);
+    const int64_t per = static_cast<int64_t>(MT) * NKS * 64;  // lanes per operator
+    if (i >= 343 * per) return;
+    const int o = static_cast<int>(i / per);
+    const int rem = static_cast<int>(i % per);
+    const int mt = rem / (NKS * 64), ks = (rem / 64) % NKS, l = rem % 64;
+    const double *src = op64 + static_cast<int64_t>(16 + o) * kk;
+    const int row = mt * 32 + (l & 31);
+    const int64_t vec = (static_cast<int64_t>(o) * MT + mt) * 2 * NKS * 64 + ks * 64 + l;
+    __bf16 *hi = frag + vec * 8;
+    __bf16 *lo = frag + (vec + NKS * 64) * 8;
+    for (int e = 0; e < 8; ++e) {
+        const int col = ks * 16 + (l >> 5) * 8 + e;
+        const double x = (row < K && col < K) ? src[static_cast<int64_t>(row) * K + col] : 0.0;
+        const __bf16 h = static_cast<__bf16>(static_cast<float>(x));
+        hi[e] = h;
+        lo[e] = static_cast<__bf16>(static_cast<float>(x - static_cast<double>(static_cast<float>(h))));
+    }
+}
+
+// One block: 32*NT targets (children of octant qt of 32*NT parents) and one
+// wavefront per 32-row m-tile of the output.  Per offset step:
+//   A (this wavefront's m-tile of T_o, hi and lo) streams from L2 straight
+//     into registers, two k-steps ahead;
+//   B (the targets' source multipoles) sits in LDS as hi/lo MFMA fragments,
+//     gathered for the next step into registers while this step computes.
+template <int P, int NT>
+__global__ __launch_bounds__(64 * mt_of((P + 1) * (P + 1))) void k_m2l_split(const int32_t *__restrict__ parents, int nparents,
+                                                                             const int32_t *__restrict__ child, const uint32_t *__restrict__ count,
+                                                                             const int32_t *__restrict__ nbr27, const float *__restrict__ M,
+                                                                             float *__restrict__ L, const bf16x8 *__restrict__ ops, int ldk) {
+    constexpr int K = (P + 1) * (P + 1);
+    constexpr int MT = mt_of(K), NKS = ks_of(K);
+    constexpr int NTH = 64 * MT;             // threads
+    constexpr int TB = 32 * NT;              // targets per block
+    constexpr int TILE = 2 * NKS * 64;       // bf16x8 vectors of one m-tile of an operator (hi and lo)
+    constexpr int CH = TB * 2 * NKS;         // 8-float chunks of the B operand per step
+    constexpr int CPT = (CH + NTH - 1) / NTH;
+    __shared__ bf16x8 sB[NKS][NT][2][64];    // [k-step][n-tile][hi/lo][lane]
+    __shared__ int16_t s_dn[189], s_qs[189], s_o[189];
+    __shared__ int32_t s_sb[27][TB];         // first child of each parent neighbour (-1: none)
+    __shared__ uint32_t s_live[7][TB];       // bit 8 dn + qs: that source cell holds particles
+    const int tid = threadIdx.x, lane = tid & 63, mt = tid >> 6;
+    const int qt = blockIdx.y;
+    const int tx = 1 - (qt & 1), ty = 1 - ((qt >> 1) & 1), tz = 1 - ((qt >> 2) & 1);
+    if (tid == 0) {  // the 189 offsets of this octant, in a fixed order
+        int st = 0;
+        for (int d = 0; d < 27; ++d) {
+            const int dx = d % 3 - 1, dy = (d / 3) % 3 - 1, dz = d / 9 - 1;
+            for (int q = 0; q < 8; ++q) {
+                const int ex = 2 * dx + (1 - (q & 1)) - tx, ey = 2 * dy + (1 - ((q >> 1) & 1)) - ty, ez = 2 * dz + (1 - ((q >> 2) & 1)) - tz;
+                if (ex >= -1 && ex <= 1 && ey >= -1 && ey <= 1 && ez >= -1 && ez <= 1) continue;  // adjacent: near field
+                s_o[st] = static_cast<int16_t>((ex + 3) + 7 * (ey + 3) + 49 * (ez + 3));
+                s_dn[st] = static_cast<int16_t>(d);
+                s_qs[st] = static_cast<int16_t>(q);
+                ++st;
+            }
+        }
+    }
+    for (int t = tid; t < TB; t += NTH) {
+        const int pi = blockIdx.x * TB + t;
+        int32_t P0 = pi < nparents ? parents[pi] : -1;
+        if (P0 >= 0 && count[child[P0] + qt] == 0) P0 = -1;
+        uint32_t live[7] = {0, 0, 0, 0, 0, 0, 0};
+        for (int dn = 0; dn < 27; ++dn) {
+            const int32_t sp = P0 >= 0 ? nbr27[27 * static_cast<int64_t>(P0) + dn] : -1;
+            const int32_t sb = sp >= 0 ? child[sp] : -1;
+            s_sb[dn][t] = sb;
+            if (sb >= 0)
+                for (int q = 0; q < 8; ++q)
+                    if (count[sb + q] > 0) live[(8 * dn + q) >> 5] |= 1u << ((8 * dn + q) & 31);
+        }
+        for (int w = 0; w < 7; ++w) s_live[w][t] = live[w];
+    }
+    __syncthreads();
+    auto source_of = [&](int st, int t) -> int32_t {
+        const int dn = s_dn[st], bit = 8 * dn + s_qs[st];
+        return ((s_live[bit >> 5][t] >> (bit & 31)) & 1u) ? s_sb[dn][t] + s_qs[st] : -1;
+    };
+    // gather the B operand of step st into registers: chunk c -> target c / (2 NKS), 8 floats at k = 8 (c % (2 NKS))
+    float bq[CPT][8];
+    auto gather = [&](int st) {
+#pragma unroll
+        for (int u = 0; u < CPT; ++u) {
+            const int c = tid + u * NTH;
+            const int t = c / (2 * NKS), k8 = c % (2 * NKS);
+            const int32_t s = (c < CH) ? source_of(st, t) : -1;
+            if (s >= 0) {
+                const float *src = M + static_cast<int64_t>(s) * ldk + 8 * k8;
+                const float4 a = *reinterpret_cast<const float4 *>(src);
+                const float4 b = *reinterpret_cast<const float4 *>(src + 4);
+                bq[u][0] = a.x; bq[u][1] = a.y; bq[u][2] = a.z; bq[u][3] = a.w;
+                bq[u][4] = b.x; bq[u][5] = b.y; bq[u][6] = b.z; bq[u][7] = b.w;
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) bq[u][e] = 0.0f;
+            }
+        }
+    };
+    // split and store as MFMA B fragments: lane (k8 % 2) * 32 + t % 32 of n-tile t / 32, k-step k8 / 2
+    auto stash = [&]() {
+#pragma unroll
+        for (int u = 0; u < CPT; ++u) {
+            const int c = tid + u * NTH;
+            if (c >= CH) continue;
+            const int t = c / (2 * NKS), k8 = c % (2 * NKS);
+            bf16x8 h, l;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const __bf16 hb = static_cast<__bf16>(bq[u][e]);
+                h[e] = hb;
+                l[e] = static_cast<__bf16>(bq[u][e] - static_cast<float>(hb));
+            }
+            const int ln = (k8 & 1) * 32 + (t & 31);
+            sB[k8 >> 1][t >> 5][0][ln] = h;
+            sB[k8 >> 1][t >> 5][1][ln] = l;
+        }
+    };
+    f32x16 acc[NT];
+#pragma unroll
+    for (int n = 0; n < NT; ++n)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[n][r] = 0.0f;
+    gather(0);
+    stash();
+    __syncthreads();
+    // A fragments of this wavefront's m-tile: flat index q = step * NKS + ks, prefetched two ahead
+    constexpr int TOT = 189 * NKS;
+    auto afrag = [&](int q, int h) -> bf16x8 {
+        const int st = q / NKS, ks = q % NKS;
+        return ops[(static_cast<int64_t>(s_o[st]) * MT + mt) * TILE + h * NKS * 64 + ks * 64 + lane];
+    };
+    bf16x8 a0h = afrag(0, 0), a0l = afrag(0, 1);
+    bf16x8 a1h = afrag(1, 0), a1l = afrag(1, 1);
+    for (int st = 0; st < 189; ++st) {
+        if (st + 1 < 189) gather(st + 1);  // in flight during this step
+#pragma unroll
+        for (int ks = 0; ks < NKS; ++ks) {
+            const int q = st * NKS + ks;
+            const bf16x8 ch = a0h, cl = a0l;
+            a0h = a1h;
+            a0l = a1l;
+            if (q + 2 < TOT) {
+                a1h = afrag(q + 2, 0);
+                a1l = afrag(q + 2, 1);
+            }
+            bf16x8 bh[NT], bl[NT];
+#pragma unroll
+            for (int n = 0; n < NT; ++n) {
+                bh[n] = sB[ks][n][0][lane];
+                bl[n] = sB[ks][n][1][lane];
+            }
+#pragma unroll
+            for (int n = 0; n < NT; ++n) acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ch, bh[n], acc[n], 0, 0, 0);
+#pragma unroll
+            for (int n = 0; n < NT; ++n) acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ch, bl[n], acc[n], 0, 0, 0);
+#pragma unroll
+            for (int n = 0; n < NT; ++n) acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(cl, bh[n], acc[n], 0, 0, 0);
+        }
+        __syncthreads();  // every wavefront is done with this step's B
+        if (st + 1 < 189) stash();
+        __syncthreads();
+    }
+    // C[8(r/4) + 4(l/32) + r%4][l%32] of n-tile n -> row of this m-tile, target 32 n + l%32
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+        const int t = 32 * n + (lane & 31);
+        const int pi = blockIdx.x * TB + t;
+        const int32_t P0 = pi < nparents ? parents[pi] : -1;
+        if (P0 < 0) continue;
+        const int32_t tgt = child[P0] + qt;
+        if (count[tgt] == 0) continue;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int row = mt * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            if (row < K) L[static_cast<int64_t>(tgt) * ldk + row] = acc[n][r];
+        }
+    }
+}
+
+template <int P>
+void launch_split(Ctx &c, const int32_t *parents, int np) {
+    constexpr int K = (P + 1) * (P + 1);
+    constexpr int NT = 2;
+    const dim3 grid(ceil_div(np, 32 * NT), 8);
+    k_m2l_split<P, NT><<<grid, 64 * mt_of(K), 0, c.stream>>>(parents, np, c.t.child, c.t.count, c.t.nbr27, c.M32, c.L32,
+                                                             reinterpret_cast<const bf16x8 *>(c.opsplit.p), c.ldk);
+    VG_LAUNCH_CHECK();
+}
+
+}  // namespace
+
+void m2l_split_operators(Ctx &c, int nvec) {
+    const int MT = mt_of(c.K), NKS = ks_of(c.K);
+    const int64_t per = static_cast<int64_t>(MT) * NKS * 64;
+    c.opsplit.ensure(static_cast<size_t>(343 * 2 * per * 8));
+    const int64_t n = std::max<int64_t>(static_cast<int64_t>(nvec) * c.K * c.K, 343 * per);
+    k_split_ops<<<ceil_div(n, 256), 256, 0, c.stream>>>(c.op64, nvec, c.K, c.op32, reinterpret_cast<__bf16 *>(c.opsplit.p), MT, NKS);
+    VG_LAUNCH_CHECK();
+}
+
+void fmm_m2l_split(Ctx &c) {
+    TreeDev &t = c.t;
+    if (t.depth < 1) return;
+    const int64_t a = t.int_off[1], b = t.int_off.back();
+    if (b <= a) return;
+    const int32_t *parents = t.int_ids.p + a;
+    const int np = static_cast<int>(b - a);
+    switch (c.p) {
+        case 1: launch_split<1>(c, parents, np); break;
+        case 2: launch_split<2>(c, parents, np); break;
+        case 3: launch_split<3>(c, parents, np); break;
+        case 4: launch_split<4>(c, parents, np); break;
+        case 5: launch_split<5>(c, parents, np); break;
+        case 6: launch_split<6>(c, parents, np); break;
+        case 7: launch_split<7>(c, parents, np); break;
+        case 8: launch_split<8>(c, parents, np); break;
+        case 9: launch_split<9>(c, parents, np); break;
+        case 10: launch_split<10>(c, parents, np); break;
+        case 11: launch_split<11>(c, parents, np); break;
+        case 12: launch_split<12>(c, parents, np); break;
+        default: throw Error(1, "split M2L supports 1 <= p <= 12");
+    }
+}
 
 }  // namespace vg

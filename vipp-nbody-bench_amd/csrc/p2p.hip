@@ -9,8 +9,8 @@
 // of the sorted particles, so each entry is a (start, count) range.
 //
 // One 256-thread block per target leaf: the ranges are flattened in LDS and
-// streamed in tiles of CH source particles; each target is shared by
-// 256/ntargets threads that split the tile, then reduce.  With a nonzero
+// streamed in tiles of CH source particles; the block's slices split each
+// tile and reduce at the end.  With a nonzero
 // softening the i == j term is exactly zero (dx = dy = dz = 0), so it needs no
 // branch; without softening a select removes it.
 #include "gravity.hpp"
@@ -20,6 +20,7 @@ namespace {
 
 constexpr int P2P_THREADS = 256;
 constexpr int MAXE = 27 * (MAXD + 1);
+constexpr uint32_t P2P_CHUNK = 4096;  // sources per work item
 
 template <class R>
 struct Vec4;
@@ -35,11 +36,119 @@ struct Vec4<double> {
 __device__ inline float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ inline double rsq(double x) { return 1.0 / sqrt(x); }
 
-template <class R, bool SELF_GUARD, bool POT>
-__global__ __launch_bounds__(P2P_THREADS) void k_p2p(const int32_t *__restrict__ leaves, const int8_t *__restrict__ level, const uint32_t *__restrict__ start,
-                                                     const uint32_t *__restrict__ count, const int32_t *__restrict__ anc, const int32_t *__restrict__ ni,
-                                                     const int32_t *__restrict__ ni_cnt, const typename Vec4<R>::T *__restrict__ xs, R eps2, R G,
-                                                     typename Vec4<R>::T *__restrict__ acc) {
+// Sources [src0, src1) of the leaf's flattened list against targets pt0 ..
+// pt0+nt-1.  out == nullptr: write the final G-scaled result to acc; else
+// write the raw partial sums to out[t].
+template <class R, bool SELF_GUARD, bool POT, int T, class V4>
+__device__ __forceinline__ void p2p_leaf(V4 *tile, V4 *red, const uint32_t *ent_start, const uint32_t *ent_off, int E, uint32_t src0, uint32_t src1,
+                                         uint32_t nt, uint32_t pt0, const V4 *__restrict__ xs, R eps2, R G, V4 *__restrict__ acc,
+                                         V4 *__restrict__ out) {
+    constexpr int CH = sizeof(R) == 4 ? 1024 : 512;
+    const int tid = threadIdx.x;
+    const int TPW = nt > 32 ? 64 : (nt > 16 ? 32 : 16);
+    const int lane = tid & 63, wave = tid >> 6;
+    const int groups = 64 / TPW;
+    const int nsl = 4 * groups;
+    const int sl = wave * groups + lane / TPW;
+    const int ti = lane % TPW;
+    V4 xi[T];
+    R ax[T], ay[T], az[T], ph[T];
+#pragma unroll
+    for (int q = 0; q < T; ++q) {
+        const uint32_t t = ti + q * TPW;
+        xi[q] = t < nt ? xs[pt0 + t] : V4{R(1e30), R(1e30), R(1e30), R(0)};
+        ax[q] = ay[q] = az[q] = ph[q] = 0;
+    }
+    for (uint32_t c0 = src0; c0 < src1; c0 += CH) {
+        const uint32_t len = min(static_cast<uint32_t>(CH), src1 - c0);
+#pragma unroll
+        for (int u = 0; u < CH / P2P_THREADS; ++u) {
+            const uint32_t f = tid + u * P2P_THREADS;
+            const uint32_t g = c0 + min(f, len - 1);
+            int lo = 0;  // last e with ent_off[e] <= g (fixed trip count: E <= 1023)
+#pragma unroll
+            for (int step = 512; step > 0; step >>= 1)
+                if (lo + step <= E && ent_off[lo + step] <= g) lo += step;
+            if (f < len) tile[f] = xs[ent_start[lo] + (g - ent_off[lo])];
+        }
+        __syncthreads();
+        // slice sl takes the contiguous range [b0, b1) of this tile
+        const uint32_t per = (len + nsl - 1) / nsl;
+        const uint32_t b0 = min(len, sl * per), b1 = min(len, b0 + per);
+        uint32_t j = b0;
+        for (; j + 4 <= b1; j += 4) {
+            V4 sj[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) sj[u] = tile[j + u];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int q = 0; q < T; ++q) {
+                    const R dx = sj[u].x - xi[q].x, dy = sj[u].y - xi[q].y, dz = sj[u].z - xi[q].z;
+                    const R r2 = eps2 + dx * dx + dy * dy + dz * dz;
+                    R rinv = rsq(r2);
+                    if (SELF_GUARD) rinv = r2 > R(0) ? rinv : R(0);
+                    const R mr = sj[u].w * rinv;
+                    const R mr3 = mr * rinv * rinv;
+                    ax[q] += mr3 * dx;
+                    ay[q] += mr3 * dy;
+                    az[q] += mr3 * dz;
+                    if (POT) ph[q] += mr;
+                }
+        }
+        for (; j < b1; ++j) {
+            const V4 sj = tile[j];
+#pragma unroll
+            for (int q = 0; q < T; ++q) {
+                const R dx = sj.x - xi[q].x, dy = sj.y - xi[q].y, dz = sj.z - xi[q].z;
+                const R r2 = eps2 + dx * dx + dy * dy + dz * dz;
+                R rinv = rsq(r2);
+                if (SELF_GUARD) rinv = r2 > R(0) ? rinv : R(0);
+                const R mr = sj.w * rinv;
+                const R mr3 = mr * rinv * rinv;
+                ax[q] += mr3 * dx;
+                ay[q] += mr3 * dy;
+                az[q] += mr3 * dz;
+                if (POT) ph[q] += mr;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < T; ++q) red[q * P2P_THREADS + tid] = V4{ax[q], ay[q], az[q], ph[q]};
+    __syncthreads();
+    for (uint32_t t = tid; t < nt; t += P2P_THREADS) {
+        // target t lives in lane-slot (q, ti) of every slice
+        const int q = static_cast<int>(t / TPW), tl = static_cast<int>(t % TPW);
+        R sx = 0, sy = 0, sz = 0, sp = 0;
+        for (int s2 = 0; s2 < nsl; ++s2) {
+            const int owner = (s2 / groups) * 64 + (s2 % groups) * TPW + tl;
+            const V4 v = red[q * P2P_THREADS + owner];
+            sx += v.x;
+            sy += v.y;
+            sz += v.z;
+            sp += v.w;
+        }
+        if (out) {
+            out[t] = V4{sx, sy, sz, sp};
+        } else {
+            if (POT && !SELF_GUARD) sp -= xs[pt0 + t].w * rsq(eps2);  // drop the m_i / eps self term
+            acc[pt0 + t] = V4{G * sx, G * sy, G * sz, sp};
+        }
+    }
+}
+
+// Lane layout: a wavefront holds 64/TPW groups of TPW lanes; the lanes of a
+// group take targets ti, ti+TPW, ... (up to TMAX per lane) and read the same
+// source at each step (an LDS broadcast).  The block's 4 x (64/TPW) groups
+// split every staged tile of sources.
+template <class R, bool SELF_GUARD, bool POT, int T>
+__global__ __launch_bounds__(P2P_THREADS) void k_p2p(const int2 *__restrict__ items, const int32_t *__restrict__ leaves, const uint32_t *__restrict__ nchunk,
+                                                     const uint32_t *__restrict__ item_off, const int8_t *__restrict__ level,
+                                                     const uint32_t *__restrict__ start, const uint32_t *__restrict__ count, const int32_t *__restrict__ anc,
+                                                     const int32_t *__restrict__ ni, const int32_t *__restrict__ ni_cnt,
+                                                     const typename Vec4<R>::T *__restrict__ xs, R eps2, R G, typename Vec4<R>::T *__restrict__ acc,
+                                                     typename Vec4<R>::T *__restrict__ partial, uint32_t pstride) {
     using V4 = typename Vec4<R>::T;
     constexpr int CH = sizeof(R) == 4 ? 1024 : 512;
     __shared__ V4 tile[CH];
@@ -47,21 +156,24 @@ __global__ __launch_bounds__(P2P_THREADS) void k_p2p(const int32_t *__restrict__
     __shared__ uint32_t ent_off[MAXE + 1];
     __shared__ int32_t an[MAXD + 1];
     __shared__ int32_t an_off[MAXD + 2];
-    __shared__ V4 red[P2P_THREADS];
+    __shared__ V4 red[P2P_THREADS * T];
 
-    const int32_t c = leaves[blockIdx.x];
+    const int2 item = items[blockIdx.x];  // (leaf index, chunk)
+    const int32_t c = leaves[item.x];
     const int lev = level[c];
     const uint32_t nt = count[c], pt0 = start[c];
     const int tid = threadIdx.x;
     if (tid <= lev) an[tid] = anc[static_cast<int64_t>(c) * (MAXD + 1) + tid];
     __syncthreads();
-    if (tid == 0) {
-        int o = 0;
-        for (int l = 0; l <= lev; ++l) {
-            an_off[l] = o;
-            o += ni_cnt[an[l]];
+    if (tid < 64) {  // entry counts per ancestor, scanned by one wavefront
+        const int cntl = tid <= lev ? ni_cnt[an[tid]] : 0;
+        int incl = cntl;
+        for (int d = 1; d < 32; d <<= 1) {
+            const int o = __shfl_up(incl, d, 64);
+            if ((tid & 63) >= d) incl += o;
         }
-        an_off[lev + 1] = o;
+        if (tid <= lev) an_off[tid] = incl - cntl;
+        if (tid == lev) an_off[lev + 1] = incl;
     }
     __syncthreads();
     const int E = an_off[lev + 1];
@@ -73,68 +185,120 @@ __global__ __launch_bounds__(P2P_THREADS) void k_p2p(const int32_t *__restrict__
         ent_off[e + 1] = count[cell];
     }
     __syncthreads();
-    if (tid == 0) {
-        uint32_t o = 0;
-        ent_off[0] = 0;
-        for (int e = 0; e < E; ++e) {
-            o += ent_off[e + 1];
-            ent_off[e + 1] = o;
+    if (tid < 64) {  // prefix sum of the entry sizes
+        uint32_t carry = 0;
+        for (int b = 0; b < E; b += 64) {
+            const uint32_t v = (b + tid < E) ? ent_off[b + tid + 1] : 0u;
+            uint32_t incl = v;
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t o = __shfl_up(incl, d, 64);
+                if (tid >= d) incl += o;
+            }
+            if (b + tid < E) ent_off[b + tid + 1] = carry + incl;
+            carry += __shfl(incl, 63, 64);
         }
+        if (tid == 0) ent_off[0] = 0;
     }
     __syncthreads();
     const uint32_t total = ent_off[E];
-    const int S = nt ? max(1, min(16, P2P_THREADS / static_cast<int>(nt))) : 1;
-    const int ti = tid % max(1u, nt), sl = tid / max(1u, nt);
-    const bool active = nt && sl < S;
-    V4 xi{};
-    if (active) xi = xs[pt0 + ti];
-    R ax = 0, ay = 0, az = 0, ph = 0;
-    for (uint32_t c0 = 0; c0 < total; c0 += CH) {
-        const uint32_t len = min(static_cast<uint32_t>(CH), total - c0);
-        for (uint32_t f = tid; f < len; f += P2P_THREADS) {
-            const uint32_t g = c0 + f;
-            int lo = 0, hi = E;  // last e with ent_off[e] <= g
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (ent_off[mid] <= g)
-                    lo = mid;
-                else
-                    hi = mid;
-            }
-            tile[f] = xs[ent_start[lo] + (g - ent_off[lo])];
+    const uint32_t src0 = static_cast<uint32_t>(item.y) * P2P_CHUNK, src1 = min(total, src0 + P2P_CHUNK);
+    V4 *out = nchunk[item.x] > 1 ? partial + static_cast<size_t>(item_off[item.x] + item.y) * pstride : nullptr;
+    if (T < 4 || nt <= 256) {
+        p2p_leaf<R, SELF_GUARD, POT, T>(tile, red, ent_start, ent_off, E, src0, src1, nt, pt0, xs, eps2, G, acc, out);
+    } else {  // leaves above 256 points: batches of 256 targets, each streaming the sources again
+        for (uint32_t tb = 0; tb < nt; tb += 256) {
+            p2p_leaf<R, SELF_GUARD, POT, 4>(tile, red, ent_start, ent_off, E, src0, src1, min(nt - tb, 256u), pt0 + tb, xs, eps2, G, acc,
+                                            out ? out + tb : nullptr);
+            __syncthreads();
         }
+    }
+}
+
+// Per leaf: number of sources in its NI chain, split into chunks of at most
+// P2P_CHUNK sources (bounded work per block), and its class by targets per lane.
+__global__ __launch_bounds__(256) void k_p2p_plan(const int32_t *__restrict__ leaves, int nleaves, const int8_t *__restrict__ level,
+                                                  const uint32_t *__restrict__ count, const int32_t *__restrict__ anc, const int32_t *__restrict__ ni,
+                                                  const int32_t *__restrict__ ni_cnt, uint32_t *__restrict__ nchunk, uint32_t *__restrict__ cls_cnt) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= nleaves) return;
+    const int32_t c = leaves[k];
+    uint32_t src = 0;
+    for (int l = 0; l <= level[c]; ++l) {
+        const int32_t a = anc[static_cast<int64_t>(c) * (MAXD + 1) + l];
+        for (int e = 0; e < ni_cnt[a]; ++e) src += count[ni[27 * static_cast<int64_t>(a) + e]];
+    }
+    const uint32_t nch = max(1u, (src + P2P_CHUNK - 1) / P2P_CHUNK);
+    nchunk[k] = nch;
+    const uint32_t nt = count[c];
+    const int cls = nt <= 64 ? 0 : (nt <= 128 ? 1 : (nt <= 192 ? 2 : 3));
+    atomicAdd(&cls_cnt[cls], nch);
+}
+
+// Exclusive scan in one block (item offsets of the leaves); total in *sum.
+__global__ __launch_bounds__(1024) void k_scan_excl(const uint32_t *__restrict__ in, int n, uint32_t *__restrict__ out, uint32_t *__restrict__ sum) {
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    const unsigned lane = threadIdx.x & 63;
+    for (int off = 0; off < n; off += 1024) {
+        const int i = off + threadIdx.x;
+        const uint32_t v = i < n ? in[i] : 0u;
+        uint32_t s = v;
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t o = __shfl_up(s, d, 64);
+            if (lane >= static_cast<unsigned>(d)) s += o;
+        }
+        if (lane == 63) wsum[threadIdx.x >> 6] = s;
         __syncthreads();
-        if (active) {
-            for (uint32_t j = sl; j < len; j += S) {
-                const V4 sj = tile[j];
-                const R dx = sj.x - xi.x, dy = sj.y - xi.y, dz = sj.z - xi.z;
-                const R r2 = dx * dx + dy * dy + dz * dz + eps2;
-                R rinv = rsq(r2);
-                if (SELF_GUARD) rinv = r2 > R(0) ? rinv : R(0);
-                const R mr = sj.w * rinv;
-                const R mr3 = mr * rinv * rinv;
-                ax += mr3 * dx;
-                ay += mr3 * dy;
-                az += mr3 * dz;
-                if (POT) ph += mr;
-            }
-        }
+        uint32_t wb = 0;
+        for (unsigned w = 0; w < (threadIdx.x >> 6); ++w) wb += wsum[w];
+        const uint32_t c0 = carry;
+        if (i < n) out[i] = c0 + wb + s - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry = c0 + wb + s;
         __syncthreads();
     }
-    red[tid] = V4{ax, ay, az, ph};
-    __syncthreads();
-    if (active && sl == 0) {
+    if (threadIdx.x == 0) *sum = carry;
+}
+
+// Items (leaf index, chunk) bucketed by class; cls_base holds the class offsets.
+__global__ __launch_bounds__(256) void k_p2p_items(const int32_t *__restrict__ leaves, int nleaves, const uint32_t *__restrict__ count,
+                                                   const uint32_t *__restrict__ nchunk, uint32_t *__restrict__ cls_fill, uint4 cls_base,
+                                                   int2 *__restrict__ items) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= nleaves) return;
+    const uint32_t nt = count[leaves[k]];
+    const int cls = nt <= 64 ? 0 : (nt <= 128 ? 1 : (nt <= 192 ? 2 : 3));
+    const uint32_t nch = nchunk[k];
+    const uint32_t base = cls == 0 ? cls_base.x : (cls == 1 ? cls_base.y : (cls == 2 ? cls_base.z : cls_base.w));
+    const uint32_t pos = base + atomicAdd(&cls_fill[cls], nch);
+    for (uint32_t q = 0; q < nch; ++q) items[pos + q] = make_int2(k, static_cast<int>(q));
+}
+
+// Sum the chunk partials of split leaves in chunk order.
+template <class R, bool SELF_GUARD, bool POT>
+__global__ __launch_bounds__(64) void k_p2p_reduce(const int32_t *__restrict__ leaves, const uint32_t *__restrict__ nchunk, const uint32_t *__restrict__ item_off,
+                                                   const uint32_t *__restrict__ start, const uint32_t *__restrict__ count,
+                                                   const typename Vec4<R>::T *__restrict__ xs, R eps2, R G, const typename Vec4<R>::T *__restrict__ partial,
+                                                   uint32_t pstride, typename Vec4<R>::T *__restrict__ acc) {
+    using V4 = typename Vec4<R>::T;
+    const int k = blockIdx.x;
+    const uint32_t nch = nchunk[k];
+    if (nch <= 1) return;
+    const int32_t c = leaves[k];
+    const uint32_t nt = count[c], pt0 = start[c];
+    for (uint32_t t = threadIdx.x; t < nt; t += 64) {
         R sx = 0, sy = 0, sz = 0, sp = 0;
-        for (int q = 0; q < S; ++q) {
-            const V4 v = red[ti + q * nt];
+        for (uint32_t q = 0; q < nch; ++q) {
+            const V4 v = partial[static_cast<size_t>(item_off[k] + q) * pstride + t];
             sx += v.x;
             sy += v.y;
             sz += v.z;
             sp += v.w;
         }
-        // the self term of the potential (m_i / eps) is removed here
-        if (POT && !SELF_GUARD) sp -= xi.w * rsq(eps2);
-        acc[pt0 + ti] = V4{G * sx, G * sy, G * sz, sp};
+        if (POT && !SELF_GUARD) sp -= xs[pt0 + t].w * rsq(eps2);
+        acc[pt0 + t] = V4{G * sx, G * sy, G * sz, sp};
     }
 }
 
@@ -154,21 +318,89 @@ void p2p_t(Ctx &c, bool potential) {
     }
     const R eps2 = static_cast<R>(c.eps * c.eps);
     const bool guard = !(eps2 > 0);
-    const dim3 grid(static_cast<unsigned>(t.nleaves));
-#define VG_P2P(G_, P_) \
-    k_p2p<R, G_, P_><<<grid, P2P_THREADS, 0, c.stream>>>(t.leaf_ids, t.level, t.start, t.count, t.anc, t.ni, t.ni_cnt, xs, eps2, static_cast<R>(c.G), acc)
-    if (guard) {
-        if (potential)
-            VG_P2P(true, true);
-        else
-            VG_P2P(true, false);
-    } else {
-        if (potential)
-            VG_P2P(false, true);
-        else
-            VG_P2P(false, false);
+    const int nl = static_cast<int>(t.nleaves);
+    c.p2p_scal.ensure(16);
+    c.p2p_nchunk.ensure(nl);
+    c.p2p_item_off.ensure(nl);
+    uint32_t *cls_cnt = c.p2p_scal.p, *cls_fill = c.p2p_scal.p + 4, *nitems_d = c.p2p_scal.p + 12;
+    VG_HIP(hipMemsetAsync(c.p2p_scal.p, 0, 16 * sizeof(uint32_t), c.stream));
+    k_p2p_plan<<<ceil_div(nl, 256), 256, 0, c.stream>>>(t.leaf_ids, nl, t.level, t.count, t.anc, t.ni, t.ni_cnt, c.p2p_nchunk, cls_cnt);
+    k_scan_excl<<<1, 1024, 0, c.stream>>>(c.p2p_nchunk, nl, c.p2p_item_off, nitems_d);
+    VG_LAUNCH_CHECK();
+    uint32_t hs[16];
+    VG_HIP(hipMemcpyAsync(hs, c.p2p_scal.p, sizeof(hs), hipMemcpyDeviceToHost, c.stream));
+    VG_HIP(hipStreamSynchronize(c.stream));
+    const uint32_t nitems = hs[12];
+    uint32_t hbase[4], acc_base = 0;
+    for (int q = 0; q < 4; ++q) {
+        hbase[q] = acc_base;
+        acc_base += hs[q];
     }
+    c.p2p_items.ensure(std::max<uint32_t>(1, nitems));
+    int2 *items = reinterpret_cast<int2 *>(c.p2p_items.p);
+    k_p2p_items<<<ceil_div(nl, 256), 256, 0, c.stream>>>(t.leaf_ids, nl, t.count, c.p2p_nchunk, cls_fill, make_uint4(hbase[0], hbase[1], hbase[2], hbase[3]),
+                                                         items);
+    VG_LAUNCH_CHECK();
+    const bool split = nitems > static_cast<uint32_t>(nl);
+    const uint32_t pstride = c.thr;
+    V4 *partial = nullptr;
+    if (split) {
+        if (sizeof(R) == 4) {
+            c.p2p_part32.ensure(4 * static_cast<size_t>(nitems) * pstride);
+            partial = reinterpret_cast<V4 *>(c.p2p_part32.p);
+        } else {
+            c.p2p_part64.ensure(4 * static_cast<size_t>(nitems) * pstride);
+            partial = reinterpret_cast<V4 *>(c.p2p_part64.p);
+        }
+    }
+    for (int cls = 0; cls < 4; ++cls) {
+        if (!hs[cls]) continue;
+        const dim3 grid(hs[cls]);
+        const int2 *it = items + hbase[cls];
+#define VG_P2P_T(G_, P_, T_)                                                                                                                         \
+    k_p2p<R, G_, P_, T_><<<grid, P2P_THREADS, 0, c.stream>>>(it, t.leaf_ids, c.p2p_nchunk, c.p2p_item_off, t.level, t.start, t.count, t.anc, t.ni, \
+                                                             t.ni_cnt, xs, eps2, static_cast<R>(c.G), acc, partial, pstride)
+#define VG_P2P(G_, P_)           \
+    do {                         \
+        if (cls == 0)            \
+            VG_P2P_T(G_, P_, 1); \
+        else if (cls == 1)       \
+            VG_P2P_T(G_, P_, 2); \
+        else if (cls == 2)       \
+            VG_P2P_T(G_, P_, 3); \
+        else                     \
+            VG_P2P_T(G_, P_, 4); \
+    } while (0)
+        if (guard) {
+            if (potential)
+                VG_P2P(true, true);
+            else
+                VG_P2P(true, false);
+        } else {
+            if (potential)
+                VG_P2P(false, true);
+            else
+                VG_P2P(false, false);
+        }
 #undef VG_P2P
+#undef VG_P2P_T
+    }
+    if (split) {
+#define VG_RED(G_, P_) \
+    k_p2p_reduce<R, G_, P_><<<nl, 64, 0, c.stream>>>(t.leaf_ids, c.p2p_nchunk, c.p2p_item_off, t.start, t.count, xs, eps2, static_cast<R>(c.G), partial, pstride, acc)
+        if (guard) {
+            if (potential)
+                VG_RED(true, true);
+            else
+                VG_RED(true, false);
+        } else {
+            if (potential)
+                VG_RED(false, true);
+            else
+                VG_RED(false, false);
+        }
+#undef VG_RED
+    }
     VG_LAUNCH_CHECK();
 }
 
