@@ -31,8 +31,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "vipp-nbody-bench_amd"))
 
 METRIC = "FMM RK4 steps/sec at N=4M; force err vs FP64 direct; learned vs plain GPU"
-PEAK_FP32_VECTOR_TFLOPS = 157.3
+PEAK_FP32_TFLOPS = 157.3
 PEAK_BF16_MFMA_TFLOPS = 2500.0
+PEAK_FP64_TFLOPS = 78.6
 
 
 def main():
@@ -45,7 +46,12 @@ def main():
     ap.add_argument("--p", type=int, default=None)
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--err-sample", type=int, default=256)
+    ap.add_argument("--err-sample", type=int, default=65536)
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="also run one device-to-device copy of this size, the known-size kernel that scales "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to HBM bytes")
+    ap.add_argument("--no-compare-plain", dest="compare_plain", action="store_false",
+                    help="skip the plain FP32 p=12 GPU FMM run reported beside the main mode")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); multi-GPU sharding is not part of this build", file=sys.stderr)
@@ -53,6 +59,9 @@ def main():
 
     import vipp_gravity as vgm
     import workloads
+
+    if args.calib_copy_mib > 0:
+        calibration_copy(args.calib_copy_mib << 20)
 
     w = dict(workloads.WORKLOADS[args.workload])
     if args.p:
@@ -66,17 +75,22 @@ def main():
     g = vgm.Gravity(n, w["s"], w["p"], mode, eps=w["eps"], G=1.0, mass=1.0 / n)
     g.upload(x, v, masses)
 
-    # accuracy on the initial state (untimed)
+    # accuracy on the initial state (untimed): FP64 direct sum on the device for
+    # a random sample, cross-checked against the CPU oracle on its first 32
     a = g.accelerations()
     cnt = g.counts()
     err = None
     if args.err_sample > 0:
+        idx = np.sort(np.random.default_rng(1).choice(n, size=min(n, args.err_sample), replace=False))
+        d = g.direct(idx)
+        e = np.linalg.norm(a[idx] - d, axis=1) / np.linalg.norm(d, axis=1)
+        err = {"median": float(np.median(e)), "p99": float(np.percentile(e, 99)), "max": float(e.max()), "sample": int(len(idx)),
+               "reference": "fp64 direct sum on the device (vipp_grav_direct)"}
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         from oracle import Oracle
-        idx = np.random.default_rng(1).choice(n, size=min(n, args.err_sample), replace=False)
-        d = Oracle("orc").direct_subset(x, idx, m=1.0 / n, G=1.0, eps=w["eps"], masses=masses)
-        e = np.linalg.norm(a[idx] - d, axis=1) / np.linalg.norm(d, axis=1)
-        err = {"median": float(np.median(e)), "p99": float(np.percentile(e, 99)), "max": float(e.max()), "sample": int(len(idx))}
+        k = min(32, len(idx))
+        dc = Oracle("orc").direct_subset(x, idx[:k], m=1.0 / n, G=1.0, eps=w["eps"], masses=masses)
+        err["direct_vs_cpu_oracle_max_rel"] = float(np.max(np.linalg.norm(d[:k] - dc, axis=1) / np.linalg.norm(dc, axis=1)))
     g.upload(x, v, masses)
 
     for _ in range(args.warmup):
@@ -88,25 +102,28 @@ def main():
     t1 = time.perf_counter()
     ph = g.phase_times()
     g.set_timing(False)
+    cnt = g.counts()  # work of the last timed evaluation
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     value = 1e3 / ms_per_step
     nev = 4 * args.steps
     per_eval = {k: v / nev for k, v in ph.items()}
+    g.close()
 
+    # the dominant single kernel: M2L is one launch per evaluation
     p = w["p"]
     m2l_flops = 2.0 * (p + 1) ** 4 * cnt["m2l_live_pairs"]
-    p2p_flops = 20.0 * cnt["p2p_pairs"]
-    m2l_peak = PEAK_BF16_MFMA_TFLOPS if mode == "split" else PEAK_FP32_VECTOR_TFLOPS
-    cand = {
-        "m2l": (m2l_flops, per_eval["m2l"], m2l_peak),
-        "p2p": (p2p_flops, per_eval["p2p"], PEAK_FP32_VECTOR_TFLOPS),
-    }
-    dom = max(cand, key=lambda k: cand[k][1])
-    fl, ms, peak = cand[dom]
-    achieved = fl / (ms * 1e-3) / 1e12
-    roofline = {"kernel": dom, "bound": "mfma" if (dom == "m2l" and mode == "split") else "valu", "achieved": achieved, "peak": peak,
-                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
-                "work_per_launch": fl, "ms_per_launch": ms}
+    ms = per_eval["m2l"]
+    split = mode == "split"
+    peak = {"split": PEAK_BF16_MFMA_TFLOPS, "f32": PEAK_FP32_TFLOPS, "f64": PEAK_FP64_TFLOPS}[mode]
+    achieved = m2l_flops / (ms * 1e-3) / 1e12
+    roofline = {"kernel": "k_m2l_split" if split else "k_m2l_plain", "bound": "mfma", "achieved": achieved, "peak": peak,
+                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": m2l_traffic(args.workload, mode),
+                "flop_per_launch": m2l_flops, "ms_per_launch": ms,
+                "work": f"2(p+1)^4 = {2 * (p + 1) ** 4} FLOP per M2L pair whose source and target hold particles"}
+
+    plain = None
+    if args.compare_plain and mode != "f32":
+        plain = plain_reference(vgm, x, v, masses, w, n, args.steps, value)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -121,13 +138,61 @@ def main():
                    "p2p": "fp64" if mode == "f64" else "fp32", "state": "fp64", "parallelism": "1 GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "vs_plain_gpu": plain,
         "phase_ms_per_eval": per_eval,
         "p2p_pairs_per_s": cnt["p2p_pairs"] / (per_eval["p2p"] * 1e-3) if per_eval["p2p"] > 0 else None,
+        "p2p_tflops": 20.0 * cnt["p2p_pairs"] / (per_eval["p2p"] * 1e-3) / 1e12 if per_eval["p2p"] > 0 else None,
         "m2l_pairs_per_s": cnt["m2l_live_pairs"] / (per_eval["m2l"] * 1e-3) if per_eval["m2l"] > 0 else None,
         "counts": cnt,
         "force_err": err,
     }
     print(json.dumps(line))
+
+
+def plain_reference(vgm, x, v, masses, w, n, steps, value):
+    """The plain GPU FMM (FP32 P2P, FP32 M2L on the vector units, same p) on the
+    same input and step count: the comparison the headline metric asks for."""
+    g = vgm.Gravity(n, w["s"], w["p"], "f32", eps=w["eps"], G=1.0, mass=1.0 / n)
+    g.upload(x, v, masses)
+    g.step(w["dt"], 1)
+    g.upload(x, v, masses)
+    t0 = time.perf_counter()
+    g.step(w["dt"], steps)
+    t1 = time.perf_counter()
+    g.close()
+    plain = steps / (t1 - t0)
+    return {"mode": "plain fp32 p2p + fp32 m2l", "steps_per_s": plain, "speedup": value / plain}
+
+
+def calibration_copy(nbytes):
+    """One hipMemcpy device-to-device of nbytes (reads and writes nbytes once)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    for buf in (a, b):
+        if hip.hipMalloc(C.byref(buf), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipMemset(a, 1, C.c_size_t(nbytes)) != 0 or hip.hipMemset(b, 0, C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMemset failed")
+        if hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) != 0 or hip.hipDeviceSynchronize() != 0:  # 3: device to device
+            raise RuntimeError("hipMemcpy failed")
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+
+
+def m2l_traffic(workload, mode):
+    """HBM bytes per M2L launch from the committed rocprofv3 FETCH_SIZE and
+    WRITE_SIZE passes (scaled by the copy-kernel calibration), if present."""
+    name = {"split": "k_m2l_split", "f32": "k_m2l_plain"}.get(mode)
+    path = os.path.join(ROOT, "profiles", f"{name}_{workload}_traffic.txt")
+    if name is None or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("hbm_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
 
 
 def cpu_baseline(x, w, n):
@@ -138,7 +203,7 @@ def cpu_baseline(x, w, n):
     if not oracle.available("ref"):
         return None
     ref = oracle.Oracle("ref")
-    per = 16
+    per = 32
     secs, frac = ref.time_sample(x, w["s"], w["p"], 1.0 / n, 1.0, w["eps"], per)
     t_eval = float(secs.sum())
     return {"value": 1.0 / (4 * t_eval), "unit": "steps/s", "cores": 1, "kind": "reference",

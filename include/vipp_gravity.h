@@ -108,6 +108,14 @@ int vipp_grav_counts(vipp_grav_ctx *ctx, double *out, int32_t n);
 int vipp_grav_phase_times(vipp_grav_ctx *ctx, double *ms, int32_t n);
 int vipp_grav_set_timing(vipp_grav_ctx *ctx, int32_t enabled);
 
+/* FP64 direct-sum accelerations of the particles idx[0..nidx) (caller
+ * indices) against all N at the current positions: the O(N^2) force of
+ * SolverRefNaive::acceleration (solver_ref_naive.hpp:103-119), with the
+ * context's G, softening and masses.  a receives 3 nidx values; pot (may be
+ * NULL) receives sum_{j != i} m_j / sqrt(r^2 + eps^2) without G.  Runs on the
+ * device; meant for force-error samples of large N. */
+int vipp_grav_direct(vipp_grav_ctx *ctx, const int64_t *idx, int64_t nidx, double *a, double *pot);
+
 /* Device pointers of the FP64 state (3n each) for callers that keep data
  * resident on the GPU; valid until destroy. */
 int vipp_grav_device_state(vipp_grav_ctx *ctx, void **x, void **v);

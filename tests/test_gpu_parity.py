@@ -214,6 +214,35 @@ def test_energy_f64(vg, orc):
     assert abs(pe - pe_ref) <= 1e-6 * abs(pe_ref), (pe, pe_ref)
 
 
+@pytest.mark.parametrize("eps,with_masses", [(1e-3, False), (0.0, True)])
+def test_direct_sum_vs_oracle(vg, orc, eps, with_masses):
+    """Device FP64 direct sum (solver_ref_naive.hpp:103-119) against the CPU
+    oracle: relative 1e-13 of |a| (sqrt/division rounding and summation order
+    differ); the sample includes duplicates-free indices at both ends."""
+    rng = np.random.default_rng(7)
+    n = 20000
+    x = rng.standard_normal((n, 3))
+    masses = rng.uniform(0.5, 1.5, n) / n if with_masses else None
+    idx = np.concatenate([[0, n - 1], rng.choice(np.arange(1, n - 1), 300, replace=False)])
+    a_ref, pot_ref = orc.direct_subset(x, idx, m=1.0 / n, G=0.7, eps=eps, masses=masses, potential=True)
+    with vg.Gravity(n, 64, 4, "f32", eps=eps, G=0.7) as gr:
+        gr.upload(x, None, masses)
+        a, pot = gr.direct(idx, potential=True)
+        a2 = gr.direct(idx[:5])
+    rel = np.linalg.norm(a - a_ref, axis=1) / np.linalg.norm(a_ref, axis=1)
+    assert rel.max() < 1e-13, rel.max()
+    assert np.abs(pot - pot_ref).max() < 1e-13 * np.abs(pot_ref).max()
+    assert np.array_equal(a2, a[:5])  # same source slicing for both calls: same bits
+
+
+def test_direct_sum_rejects_bad_index(vg):
+    with vg.Gravity(100, 4, 4, "f32") as gr:
+        gr.upload(np.random.default_rng(0).random((100, 3)))
+        with pytest.raises(vg.GravityError) as e:
+            gr.direct([0, 100])
+        assert e.value.code == vg.E_INVALID
+
+
 # -------------------------------------------------------------- error paths --
 def test_non_finite_input_rejected(vg):
     x = np.random.default_rng(0).random((100, 3))

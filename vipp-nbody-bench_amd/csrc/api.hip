@@ -320,6 +320,17 @@ int vipp_grav_energy(vipp_grav_ctx *ctx, double *kinetic, double *potential) {
 }
 
 
+int vipp_grav_direct(vipp_grav_ctx *ctx, const int64_t *idx, int64_t nidx, double *a, double *pot) {
+    return guarded(ctx, [&] {
+        need(ctx != nullptr, VIPP_GRAV_E_INVALID, "null context");
+        need(nidx >= 0 && (nidx == 0 || (idx && a)), VIPP_GRAV_E_INVALID, "null argument");
+        vg::Ctx &c = ctx->c;
+        need(c.uploaded, VIPP_GRAV_E_STATE, "direct before upload");
+        VG_HIP(hipSetDevice(c.device));
+        vg::direct_subset(c, idx, nidx, a, pot);
+    });
+}
+
 int vipp_grav_tree_info(vipp_grav_ctx *ctx, int64_t *ncells, int64_t *nleaves, int32_t *depth) {
     return guarded(ctx, [&] {
         need(ctx != nullptr, VIPP_GRAV_E_INVALID, "null context");

@@ -127,6 +127,7 @@ void rk4_stage(Ctx &c, int stage, double dt);
 void count_work(Ctx &c);
 double kinetic_energy(Ctx &c);
 double potential_energy(Ctx &c);
+void direct_subset(Ctx &c, const int64_t *idx, int64_t nidx, double *a, double *pot);  // FP64 O(N nidx) oracle forces
 
 void phase_begin(Ctx &c, Phase p);
 void phase_end(Ctx &c, Phase p);

@@ -59,20 +59,26 @@ __global__ __launch_bounds__(256) void k_split_ops(const double *__restrict__ op
     }
 }
 
-// One block: 32*NT targets (children of octant qt of 32*NT parents) and one
-// wavefront per 32-row m-tile of the output.  Per offset step:
-//   A (this wavefront's m-tile of T_o, hi and lo) streams from L2 straight
+// One block: 32*NT targets (children of octant qt of 32*NT parents) and NG
+// wavefronts per 32-row m-tile of the output, each owning NT/NG n-tiles of 32
+// targets.  Per offset step:
+//   A (the wavefront's m-tile of T_o, hi and lo) streams from L2 straight
 //     into registers, two k-steps ahead;
 //   B (the targets' source multipoles) sits in LDS as hi/lo MFMA fragments,
 //     gathered for the next step into registers while this step computes.
-template <int P, int NT>
-__global__ __launch_bounds__(64 * mt_of((P + 1) * (P + 1))) void k_m2l_split(const int32_t *__restrict__ parents, int nparents,
+// The B fragment of lane l is stored in slot l ^ (k8 & 15), k8 = 2 ks + l / 32,
+// so that both the row-wise stores and the fragment loads spread over the
+// LDS banks.
+template <int P, int NT, int NG>
+__global__ __launch_bounds__(64 * NG * mt_of((P + 1) * (P + 1))) void k_m2l_split(const int32_t *__restrict__ parents, int nparents,
                                                                              const int32_t *__restrict__ child, const uint32_t *__restrict__ count,
                                                                              const int32_t *__restrict__ nbr27, const float *__restrict__ M,
                                                                              float *__restrict__ L, const bf16x8 *__restrict__ ops, int ldk) {
     constexpr int K = (P + 1) * (P + 1);
     constexpr int MT = mt_of(K), NKS = ks_of(K);
-    constexpr int NTH = 64 * MT;             // threads
+    constexpr int NTH = 64 * MT * NG;        // threads
+    constexpr int NW = NT / NG;              // n-tiles per wavefront
+    static_assert(NT % NG == 0, "n-tiles must split evenly over the wavefronts of an m-tile");
     constexpr int TB = 32 * NT;              // targets per block
     constexpr int TILE = 2 * NKS * 64;       // bf16x8 vectors of one m-tile of an operator (hi and lo)
     constexpr int CH = TB * 2 * NKS;         // 8-float chunks of the B operand per step
@@ -81,25 +87,52 @@ __global__ __launch_bounds__(64 * mt_of((P + 1) * (P + 1))) void k_m2l_split(con
     __shared__ int16_t s_dn[189], s_qs[189], s_o[189];
     __shared__ int32_t s_sb[27][TB];         // first child of each parent neighbour (-1: none)
     __shared__ uint32_t s_live[7][TB];       // bit 8 dn + qs: that source cell holds particles
-    const int tid = threadIdx.x, lane = tid & 63, mt = tid >> 6;
-    const int qt = blockIdx.y;
-    const int tx = 1 - (qt & 1), ty = 1 - ((qt >> 1) & 1), tz = 1 - ((qt >> 2) & 1);
-    if (tid == 0) {  // the 189 offsets of this octant, in a fixed order
+    __shared__ float s_sgn[16 * NKS];        // D_qt: sign of each coefficient under this octant's mirror
+    // XCD-aware placement: blocks go round-robin to the 8 XCDs, so XCD x takes
+    // the contiguous run of work items [x per, (x+1) per); the 8 octant blocks
+    // of a parent chunk, which read the same source multipoles and operators,
+    // run side by side on one XCD and share its L2.
+    const int nchunks = (nparents + TB - 1) / TB;
+    const int w = (blockIdx.x & 7) * nchunks + (blockIdx.x >> 3);
+    if (w >= 8 * nchunks) return;
+    const int chunk = w >> 3, qt = w & 7;
+    const int tid = threadIdx.x, lane = tid & 63, mt = (tid >> 6) % MT, n0 = ((tid >> 6) / MT) * NW;
+    // Every block walks the 189 offsets e of octant 0 in one fixed order.  For
+    // octant qt the geometry is octant 0's mirrored in the axes where qt has a
+    // bit set (parent neighbour d -> mirror(d), source octant q -> q ^ qt), and
+    // the mirrored operator is D_qt T_e D_qt with D_qt diagonal +-1 (x -> -x:
+    // (-1)^m on cosine, -(-1)^m on sine coefficients; y -> -y: -1 on sine;
+    // z -> -z: (-1)^(n+m)).  So L = D_qt sum_e T_e (D_qt M): the signs go on the
+    // gathered multipoles and on the result, and all blocks share 189 operators.
+    if (tid == 0) {
         int st = 0;
         for (int d = 0; d < 27; ++d) {
             const int dx = d % 3 - 1, dy = (d / 3) % 3 - 1, dz = d / 9 - 1;
             for (int q = 0; q < 8; ++q) {
-                const int ex = 2 * dx + (1 - (q & 1)) - tx, ey = 2 * dy + (1 - ((q >> 1) & 1)) - ty, ez = 2 * dz + (1 - ((q >> 2) & 1)) - tz;
+                const int ex = 2 * dx - (q & 1), ey = 2 * dy - ((q >> 1) & 1), ez = 2 * dz - ((q >> 2) & 1);
                 if (ex >= -1 && ex <= 1 && ey >= -1 && ey <= 1 && ez >= -1 && ez <= 1) continue;  // adjacent: near field
+                const int mx = (qt & 1) ? -dx : dx, my = (qt & 2) ? -dy : dy, mz = (qt & 4) ? -dz : dz;
                 s_o[st] = static_cast<int16_t>((ex + 3) + 7 * (ey + 3) + 49 * (ez + 3));
-                s_dn[st] = static_cast<int16_t>(d);
-                s_qs[st] = static_cast<int16_t>(q);
+                s_dn[st] = static_cast<int16_t>((mx + 1) + 3 * (my + 1) + 9 * (mz + 1));
+                s_qs[st] = static_cast<int16_t>(q ^ qt);
                 ++st;
             }
         }
     }
+    for (int k = tid; k < 16 * NKS; k += NTH) {
+        int nn = 0;  // k = n^2 + r: degree n, r = 0 (m = 0), 2m - 1 (cosine) or 2m (sine)
+        while ((nn + 1) * (nn + 1) <= k) ++nn;
+        const int r = k - nn * nn;
+        const int m = (r + 1) >> 1;
+        const bool im = r > 0 && (r & 1) == 0;
+        int sg = 1;
+        if (qt & 1) sg *= ((m & 1) ? -1 : 1) * (im ? -1 : 1);
+        if ((qt & 2) && im) sg = -sg;
+        if ((qt & 4) && ((nn + m) & 1)) sg = -sg;
+        s_sgn[k] = static_cast<float>(sg);
+    }
     for (int t = tid; t < TB; t += NTH) {
-        const int pi = blockIdx.x * TB + t;
+        const int pi = chunk * TB + t;
         int32_t P0 = pi < nparents ? parents[pi] : -1;
         if (P0 >= 0 && count[child[P0] + qt] == 0) P0 = -1;
         uint32_t live[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -148,18 +181,19 @@ __global__ __launch_bounds__(64 * mt_of((P + 1) * (P + 1))) void k_m2l_split(con
             bf16x8 h, l;
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
-                const __bf16 hb = static_cast<__bf16>(bq[u][e]);
+                const float b = bq[u][e] * s_sgn[8 * k8 + e];
+                const __bf16 hb = static_cast<__bf16>(b);
                 h[e] = hb;
-                l[e] = static_cast<__bf16>(bq[u][e] - static_cast<float>(hb));
+                l[e] = static_cast<__bf16>(b - static_cast<float>(hb));
             }
-            const int ln = (k8 & 1) * 32 + (t & 31);
+            const int ln = ((k8 & 1) * 32 + (t & 31)) ^ (k8 & 15);
             sB[k8 >> 1][t >> 5][0][ln] = h;
             sB[k8 >> 1][t >> 5][1][ln] = l;
         }
     };
-    f32x16 acc[NT];
+    f32x16 acc[NW];
 #pragma unroll
-    for (int n = 0; n < NT; ++n)
+    for (int n = 0; n < NW; ++n)
 #pragma unroll
         for (int r = 0; r < 16; ++r) acc[n][r] = 0.0f;
     gather(0);
@@ -185,18 +219,19 @@ __global__ __launch_bounds__(64 * mt_of((P + 1) * (P + 1))) void k_m2l_split(con
                 a1h = afrag(q + 2, 0);
                 a1l = afrag(q + 2, 1);
             }
-            bf16x8 bh[NT], bl[NT];
+            const int slot = lane ^ ((2 * ks + (lane >> 5)) & 15);
+            bf16x8 bh[NW], bl[NW];
 #pragma unroll
-            for (int n = 0; n < NT; ++n) {
-                bh[n] = sB[ks][n][0][lane];
-                bl[n] = sB[ks][n][1][lane];
+            for (int n = 0; n < NW; ++n) {
+                bh[n] = sB[ks][n0 + n][0][slot];
+                bl[n] = sB[ks][n0 + n][1][slot];
             }
 #pragma unroll
-            for (int n = 0; n < NT; ++n) acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ch, bh[n], acc[n], 0, 0, 0);
+            for (int n = 0; n < NW; ++n) acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ch, bh[n], acc[n], 0, 0, 0);
 #pragma unroll
-            for (int n = 0; n < NT; ++n) acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ch, bl[n], acc[n], 0, 0, 0);
+            for (int n = 0; n < NW; ++n) acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ch, bl[n], acc[n], 0, 0, 0);
 #pragma unroll
-            for (int n = 0; n < NT; ++n) acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(cl, bh[n], acc[n], 0, 0, 0);
+            for (int n = 0; n < NW; ++n) acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(cl, bh[n], acc[n], 0, 0, 0);
         }
         __syncthreads();  // every wavefront is done with this step's B
         if (st + 1 < 189) stash();
@@ -204,9 +239,9 @@ __global__ __launch_bounds__(64 * mt_of((P + 1) * (P + 1))) void k_m2l_split(con
     }
     // C[8(r/4) + 4(l/32) + r%4][l%32] of n-tile n -> row of this m-tile, target 32 n + l%32
 #pragma unroll
-    for (int n = 0; n < NT; ++n) {
-        const int t = 32 * n + (lane & 31);
-        const int pi = blockIdx.x * TB + t;
+    for (int n = 0; n < NW; ++n) {
+        const int t = 32 * (n0 + n) + (lane & 31);
+        const int pi = chunk * TB + t;
         const int32_t P0 = pi < nparents ? parents[pi] : -1;
         if (P0 < 0) continue;
         const int32_t tgt = child[P0] + qt;
@@ -214,7 +249,7 @@ __global__ __launch_bounds__(64 * mt_of((P + 1) * (P + 1))) void k_m2l_split(con
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int row = mt * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            if (row < K) L[static_cast<int64_t>(tgt) * ldk + row] = acc[n][r];
+            if (row < K) L[static_cast<int64_t>(tgt) * ldk + row] = acc[n][r] * s_sgn[row];
         }
     }
 }
@@ -222,9 +257,9 @@ __global__ __launch_bounds__(64 * mt_of((P + 1) * (P + 1))) void k_m2l_split(con
 template <int P>
 void launch_split(Ctx &c, const int32_t *parents, int np) {
     constexpr int K = (P + 1) * (P + 1);
-    constexpr int NT = 2;
-    const dim3 grid(ceil_div(np, 32 * NT), 8);
-    k_m2l_split<P, NT><<<grid, 64 * mt_of(K), 0, c.stream>>>(parents, np, c.t.child, c.t.count, c.t.nbr27, c.M32, c.L32,
+    constexpr int NT = 4, NG = 2;  // 128 targets, 12 wavefronts: 3 per SIMD
+    const unsigned grid = 8 * ceil_div(np, 32 * NT);  // one block per (parent chunk, child octant)
+    k_m2l_split<P, NT, NG><<<grid, 64 * NG * mt_of(K), 0, c.stream>>>(parents, np, c.t.child, c.t.count, c.t.nbr27, c.M32, c.L32,
                                                              reinterpret_cast<const bf16x8 *>(c.opsplit.p), c.ldk);
     VG_LAUNCH_CHECK();
 }

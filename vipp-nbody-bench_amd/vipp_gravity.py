@@ -69,6 +69,7 @@ def _declare(lib):
         "vipp_grav_phase_times": [vp, vp, i32],
         "vipp_grav_set_timing": [vp, i32],
         "vipp_grav_device_state": [vp, P(vp), P(vp)],
+        "vipp_grav_direct": [vp, vp, i64, vp, vp],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -171,6 +172,14 @@ class Gravity:
         a = np.zeros(3 * self.n) if copy else None
         self._check(self.lib.vipp_grav_accelerations(self.h, _ptr(a)))
         return None if a is None else a.reshape(-1, 3)
+
+    def direct(self, idx, potential=False):
+        """FP64 direct-sum accelerations (and potentials) of particles idx on the device."""
+        idx = np.ascontiguousarray(idx, dtype=np.int64)
+        a = np.zeros(3 * len(idx))
+        pot = np.zeros(len(idx)) if potential else None
+        self._check(self.lib.vipp_grav_direct(self.h, _ptr(idx), len(idx), _ptr(a), _ptr(pot)))
+        return (a.reshape(-1, 3), pot) if potential else a.reshape(-1, 3)
 
     def energy(self):
         ke, pe = C.c_double(), C.c_double()
