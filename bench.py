@@ -103,6 +103,13 @@ def main():
     ph = g.phase_times()
     g.set_timing(False)
     cnt = g.counts()  # work of the last timed evaluation
+    # host-buffer round trip as the reference's preprocess/solve/postprocess
+    # sees it: upload x, v; one step; download x, v (not the headline value)
+    t2 = time.perf_counter()
+    g.upload(x, v, masses)
+    g.step(w["dt"], 1)
+    g.download()
+    pcie_steps = 1.0 / (time.perf_counter() - t2)
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     value = 1e3 / ms_per_step
     nev = 4 * args.steps
@@ -139,6 +146,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "vs_plain_gpu": plain,
+        "host_roundtrip_steps_per_s": pcie_steps,
         "phase_ms_per_eval": per_eval,
         "p2p_pairs_per_s": cnt["p2p_pairs"] / (per_eval["p2p"] * 1e-3) if per_eval["p2p"] > 0 else None,
         "p2p_tflops": 20.0 * cnt["p2p_pairs"] / (per_eval["p2p"] * 1e-3) / 1e12 if per_eval["p2p"] > 0 else None,

@@ -2,8 +2,12 @@
 declared in include/vipp_gravity.h, and validates its configuration before
 touching a device (problem.hpp:179-184 rules)."""
 import ctypes
+import os
+import subprocess
 
 import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def test_library_exports_header_symbols():
@@ -34,3 +38,18 @@ def _create_raw(vgm, n, s, p):
     h = ctypes.c_void_p()
     rc = lib.vipp_grav_create(ctypes.byref(cfg), ctypes.byref(h))
     raise vgm.GravityError(rc, lib.vipp_grav_last_error(None).decode())
+
+
+def test_cpp_wrapper_compiles_cleanly():
+    """include/gravity/solver_gpu.hpp with the test's own Problem / Vector types:
+    no warnings under -Wall -Wextra."""
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(ROOT, "include"),
+                        os.path.join(ROOT, "tests", "cpp", "solver_gpu_test.cpp")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_c_header_is_plain_c():
+    """The C ABI header compiles as C99 (no C++ in the signatures)."""
+    r = subprocess.run(["gcc", "-std=c99", "-fsyntax-only", "-Wall", "-Wextra", "-Werror", "-x", "c", os.path.join(ROOT, "include", "vipp_gravity.h")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

@@ -42,7 +42,24 @@ def build(jobs: int = 8) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_cpp_test()
     return LIB
+
+
+ROOT = os.path.dirname(HERE)
+CPP_TEST = os.path.join(OBJ, "solver_gpu_test")
+
+
+def build_cpp_test() -> str:
+    """The C++ wrapper's test program (tests/cpp/solver_gpu_test.cpp), host code
+    linked against the library."""
+    src = os.path.join(ROOT, "tests", "cpp", "solver_gpu_test.cpp")
+    cmd = [os.environ.get("CXX", "g++"), "-std=c++17", "-O2", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include"), src,
+           "-L", os.path.dirname(LIB), "-lvipp_gravity", "-Wl,-rpath,$ORIGIN/../lib", "-o", CPP_TEST]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"C++ wrapper test build failed:\n{r.stderr}")
+    return CPP_TEST
 
 
 if __name__ == "__main__":
