@@ -8,9 +8,13 @@
 // entries of its ancestors and itself.  Every subtree is one contiguous range
 // of the sorted particles, so each entry is a (start, count) range.
 //
-// One 256-thread block per target leaf: the ranges are flattened in LDS and
-// streamed in tiles of CH source particles; the block's slices split each
-// tile and reduce at the end.  With a nonzero
+// Work items: a plan kernel counts each non-empty leaf's sources; every leaf
+// is cut into chunks of at most P2P_CHUNK sources and each (leaf, chunk) pair
+// is one 256-thread block, so leaves with long inherited lists spread over
+// many blocks.  A block flattens its ranges in LDS, streams the chunk's sources
+// through LDS tiles and gives each lane T = 1..3 targets by leaf-size class.
+// A leaf with one chunk writes its result; split leaves write partial sums
+// that a reduce kernel adds in chunk order (deterministic).  With a nonzero
 // softening the i == j term is exactly zero (dx = dy = dz = 0), so it needs no
 // branch; without softening a select removes it.
 #include "gravity.hpp"
