@@ -98,7 +98,8 @@ int vipp_grav_export_expansions(vipp_grav_ctx *ctx, double *M, double *L);
  * reference sums them, self pairs excluded), [1] M2L cell pairs (every AP
  * entry of every cell at level >= 2, as the reference evaluates them),
  * [2] cells, [3] non-empty leaves, [4] depth, [5] M2L pairs whose source and
- * target both hold particles (the pairs that carry work). */
+ * target both hold particles (the pairs that carry work), [6] leaves that took
+ * the learned closure's fallback. */
 int vipp_grav_counts(vipp_grav_ctx *ctx, double *out, int32_t n);
 
 /* Per-phase device milliseconds accumulated since the last reset, measured with
@@ -115,6 +116,52 @@ int vipp_grav_set_timing(vipp_grav_ctx *ctx, int32_t enabled);
  * NULL) receives sum_{j != i} m_j / sqrt(r^2 + eps^2) without G.  Runs on the
  * device; meant for force-error samples of large N. */
 int vipp_grav_direct(vipp_grav_ctx *ctx, const int64_t *idx, int64_t nidx, double *a, double *pot);
+
+/* ---- Learned closure -------------------------------------------------------
+ * A low-order FMM (p >= 4, FP32 expansions) whose leaf local expansions are
+ * corrected before L2P by a small network: 28 inputs (the leaf's first 25
+ * scaled local coefficients, orders <= 4, divided by their norm, then
+ * level / 16, 2^-level and the distance of the leaf centre from the root
+ * centre in root widths) -> 128 -> 128 (GELU) -> 25 outputs (the residual of
+ * those 25 coefficients against an order-12 evaluation, in the same
+ * normalisation).  FP16 weights and activations with FP32 accumulation in the
+ * forward pass (MFMA), FP32 backward pass and Adam.  Parameters are one FP32
+ * vector of VIPP_MLP_NPARAMS values, row-major [out][in]: W1 (128 x 28), b1,
+ * W2 (128 x 128), b2, W3 (25 x 128), b3.  No counterpart in the reference. */
+#define VIPP_MLP_IN 28
+#define VIPP_MLP_HIDDEN 128
+#define VIPP_MLP_OUT 25
+#define VIPP_MLP_NPARAMS 23449
+
+/* y (n x 25) = network(x (n x 28)) on the given device. */
+int vipp_mlp_forward(int32_t device, const float *params, const float *x, int64_t n, float *y);
+/* Mean squared error over the n x 25 outputs against t, each sample weighted
+ * by w (NULL: all 1; the loss is sum_j w_j |y_j - t_j|^2 / (25 sum_j w_j)),
+ * and its gradient with respect to the parameters (VIPP_MLP_NPARAMS values). */
+int vipp_mlp_loss_grad(int32_t device, const float *params, const float *x, const float *t, const float *w, int64_t n, float *loss,
+                       float *grad);
+/* Adam (beta 0.9 / 0.999, eps 1e-8) on that loss, minibatches of `batch` in an
+ * order shuffled per epoch from `seed`, learning rate decayed from lr to
+ * 0.05 lr by a half cosine over the epochs.  params is updated in place;
+ * epoch_loss (may be NULL) receives the weighted mean loss of each epoch. */
+int vipp_mlp_train(int32_t device, float *params, const float *x, const float *t, const float *w, int64_t n, uint32_t epochs, uint32_t batch,
+                   float lr, uint64_t seed, float *epoch_loss);
+
+/* The closure's view of the last evaluation, one row per non-empty leaf in the
+ * library's leaf order (the same for two contexts holding the same positions):
+ * x (n x 28) network inputs, norm (n) the norm of the first 25 scaled local
+ * coefficients, l25 (n x 25) those coefficients, level (n).  Leaves above
+ * level 2 (no far field) have norm 0.  Any output pointer may be NULL; with all
+ * NULL only *n is set.  Needs precision f32 or split and p >= 4. */
+int vipp_grav_leaf_inputs(vipp_grav_ctx *ctx, int64_t *n, float *x, float *norm, float *l25, int32_t *level);
+/* Enable the closure with the given parameters (NULL disables it).  io (81
+ * values, or NULL for the identity) standardises the network's input and
+ * scales its output: the network sees (x - io[0..27]) * io[28..55] and the
+ * correction is |L~_25| * io[56..80] * network(x).  Leaves whose predicted
+ * relative correction |io[56..80] * network(x)| exceeds `bound` take the
+ * order-p_fallback far field instead (INFINITY: never); they are counted in
+ * vipp_grav_counts()[6]. */
+int vipp_grav_set_closure(vipp_grav_ctx *ctx, const float *params, const float *io, double bound, uint32_t p_fallback);
 
 /* Device pointers of the FP64 state (3n each) for callers that keep data
  * resident on the GPU; valid until destroy. */

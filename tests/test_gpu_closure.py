@@ -1,0 +1,188 @@
+"""The learned closure on the device: the network's forward pass, gradient and
+Adam step against the CPU oracle (oracle/mlp.py), training, the closure's
+inputs, and what the trained network gains over the low-order FMM without it.
+
+Tolerances: forward outputs within 2e-3 of the largest output (the device
+rounds weights and activations to FP16 where the oracle does, but accumulates
+in FP32, so an activation can round to the neighbouring FP16 value);
+gradients within 1e-3 relative (L2, per tensor); the first Adam step equal
+to the oracle's within 1e-4 lr where |g| > 1e-5 (far above Adam's eps)."""
+import numpy as np
+import pytest
+
+import mlp as M
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def vg():
+    import vipp_gravity
+    return vipp_gravity
+
+
+@pytest.fixture(scope="module")
+def cl():
+    import closure
+    return closure
+
+
+def _params(seed, scale=1.0):
+    rng = np.random.default_rng(seed)
+    p = rng.standard_normal(M.NPAR) * 0.15 * scale
+    return p.astype(np.float32)
+
+
+@pytest.mark.parametrize("n", [1, 15, 1000])
+def test_mlp_forward_vs_oracle(vg, n):
+    rng = np.random.default_rng(n)
+    p = _params(1)
+    x = rng.standard_normal((n, M.IN)).astype(np.float32)
+    y = vg.mlp_forward(p, x)
+    ref, _ = M.forward(p, x, half=True)
+    assert np.abs(y - ref).max() <= 2e-3 * max(1.0, np.abs(ref).max()), np.abs(y - ref).max()
+    # a transposed or misplaced operand changes outputs by O(1): the bulk agrees far tighter
+    assert np.median(np.abs(y - ref)) <= 1e-5 * max(1.0, np.abs(ref).max())
+
+
+def test_mlp_grad_vs_oracle(vg):
+    rng = np.random.default_rng(3)
+    p = _params(2)
+    x = rng.standard_normal((300, M.IN)).astype(np.float32)
+    t = rng.standard_normal((300, M.OUT)).astype(np.float32)
+    loss, g = vg.mlp_loss_grad(p, x, t)
+    lref, gref = M.loss_grad(p, x, t, half=True)
+    assert abs(loss - lref) <= 1e-5 * lref
+    for a, b in [(M.OW1, M.OB1), (M.OB1, M.OW2), (M.OW2, M.OB2), (M.OB2, M.OW3), (M.OW3, M.OB3), (M.OB3, M.NPAR)]:
+        rel = np.linalg.norm(g[a:b] - gref[a:b]) / np.linalg.norm(gref[a:b])
+        assert rel < 1e-3, (a, b, rel)
+
+
+def test_mlp_weighted_grad_vs_oracle(vg):
+    rng = np.random.default_rng(8)
+    p = _params(9)
+    x = rng.standard_normal((200, M.IN)).astype(np.float32)
+    t = rng.standard_normal((200, M.OUT)).astype(np.float32)
+    w = rng.uniform(0.0, 3.0, 200).astype(np.float32)
+    w[:20] = 0.0  # zero-weight samples contribute nothing
+    loss, g = vg.mlp_loss_grad(p, x, t, w)
+    lref, gref = M.loss_grad(p, x, t, half=True, w=w)
+    assert abs(loss - lref) <= 1e-5 * lref
+    assert np.linalg.norm(g - gref) / np.linalg.norm(gref) < 1e-3
+    l2, g2 = vg.mlp_loss_grad(p, x[20:], t[20:], w[20:])
+    assert l2 == pytest.approx(loss, rel=1e-5) and np.allclose(g2, g, rtol=1e-4, atol=1e-6 * np.abs(g).max())
+
+
+def test_mlp_first_adam_step_vs_oracle(vg):
+    rng = np.random.default_rng(4)
+    p = _params(5)
+    x = rng.standard_normal((256, M.IN)).astype(np.float32)
+    t = rng.standard_normal((256, M.OUT)).astype(np.float32)
+    lr = 1e-3
+    p1, hist = vg.mlp_train(p, x, t, epochs=1, batch=256, lr=lr, seed=0)  # one full batch: one step at lr
+    _, g = M.loss_grad(p, x, t, half=True)
+    ref, _, _ = M.adam_step(p.astype(np.float64), g, np.zeros(M.NPAR), np.zeros(M.NPAR), 1, lr)
+    # where |g| is near Adam's eps (1e-8) the step depends on the gradient's last bits
+    live = np.abs(g) > 1e-5
+    assert live.mean() > 0.5
+    assert np.abs((p1 - ref)[live]).max() <= 1e-4 * lr + 1e-7
+    assert hist[0] == pytest.approx(M.loss_grad(p, x, t, half=True)[0], rel=1e-5)
+
+
+def test_mlp_training_reduces_loss(vg, cl):
+    rng = np.random.default_rng(6)
+    x = rng.standard_normal((4000, M.IN)).astype(np.float32)
+    A = rng.standard_normal((M.IN, M.OUT)) / np.sqrt(M.IN)
+    t = (np.sin(x @ A) * 0.5).astype(np.float32)
+    loss0, _ = vg.mlp_loss_grad(cl.init_params(1), x, t)
+    p, hist = vg.mlp_train(cl.init_params(1), x, t, 30, 256, 3e-3, seed=1)
+    assert hist[-1] < 0.2 * loss0, (loss0, hist)
+    loss, _ = vg.mlp_loss_grad(p, x, t)
+    assert loss == pytest.approx(M.loss_grad(p, x, t, half=True)[0], rel=1e-4)
+    p2, hist2 = vg.mlp_train(cl.init_params(1), x, t, 30, 256, 3e-3, seed=1)
+    assert np.array_equal(p, p2) and np.array_equal(hist, hist2)  # deterministic
+
+
+def test_leaf_inputs_and_order(vg, orc):
+    x, _, _ = orc.problem_initialize(30000, 1, 1e-3)
+    with vg.Gravity(len(x), 32, 4, "split", eps=1e-3) as lo, vg.Gravity(len(x), 32, 12, "split", eps=1e-3) as hi:
+        lo.upload(x)
+        hi.upload(x)
+        a, b = lo.leaf_inputs(), hi.leaf_inputs()
+        t = lo.tree()
+    assert np.array_equal(a["level"], b["level"])
+    assert len(a["level"]) == np.count_nonzero(np.diff(t.pt_off))  # one row per non-empty leaf
+    far = a["level"] >= 2
+    assert np.all(a["norm"][~far] == 0) and np.all(a["norm"][far] > 0)
+    np.testing.assert_allclose(a["x"][far, :25] * a["norm"][far, None], a["l25"][far], rtol=1e-5, atol=1e-6 * a["norm"].max())
+    assert np.allclose(a["x"][:, 25], a["level"] / 16) and np.allclose(a["x"][:, 26], 2.0 ** -a["level"])
+    # the order-12 leaf coefficients of orders <= 4 are close to the order-4 ones: what the closure learns is small
+    rel = np.linalg.norm(b["l25"][far] - a["l25"][far], axis=1) / a["norm"][far]
+    assert np.median(rel) < 0.1
+
+
+def _forces(vg, x, p, eps, params=None, io=None, bound=float("inf")):
+    with vg.Gravity(len(x), 64, p, "split", eps=eps) as g:
+        g.upload(x)
+        if params is not None:
+            g.set_closure(params, io, bound)
+        return g.accelerations(), g.counts()
+
+
+@pytest.fixture(scope="module")
+def trained(vg, cl, orc):
+    """Closure trained on pairs from three 2^20-particle Plummer spheres (seeds 11-13)."""
+    eps = 1e-3
+    X, T = [], []
+    for seed in (11, 12, 13):
+        xs, _, _ = orc.problem_initialize(1 << 20, seed, eps)
+        a, b = cl.training_pairs(xs, 64, eps)
+        X.append(a)
+        T.append(b)
+    X, T = np.concatenate(X), np.concatenate(T)
+    params, io, hist = cl.train(X, T, epochs=40, batch=256, lr=2e-3, seed=0)
+    return params, io, hist, X
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)
+
+
+def test_closure_gain_over_plain_low_order(vg, cl, orc, trained):
+    """Ablation: a p = 4 FMM with the trained network against the same p = 4 FMM
+    without it, both judged against the order-12 FMM, on positions the network
+    never saw.  The network must shrink the median difference to order 12."""
+    params, io, hist, _ = trained
+    assert hist[-1] < 0.8 * hist[0]
+    eps = 1e-3
+    x, _, _ = orc.problem_initialize(1 << 20, 14, eps)
+    a12, _ = _forces(vg, x, 12, eps)
+    a4, _ = _forces(vg, x, 4, eps)
+    ac, cnt = _forces(vg, x, 4, eps, params, io)
+    e4, ec = _rel(a4, a12), _rel(ac, a12)
+    print(f"median rel. difference to p=12: p=4 {np.median(e4):.3e}, p=4 + closure {np.median(ec):.3e}")
+    assert cnt["fallback_leaves"] == 0
+    assert np.median(ec) < 0.95 * np.median(e4)
+
+
+def test_closure_fallback(vg, cl, orc, trained):
+    """Bound 0 sends every leaf with a far field to order 12: the result must be
+    the order-12 FMM's far field (same tree, same order-12 operators) up to FP32
+    summation order; a bound for 20% of the leaves lands between."""
+    params, io, _, X = trained
+    eps = 1e-3
+    x, _, _ = orc.problem_initialize(1 << 20, 15, eps)
+    a12, _ = _forces(vg, x, 12, eps)
+    a4, _ = _forces(vg, x, 4, eps)
+    af, cnt = _forces(vg, x, 4, eps, params, io, bound=0.0)
+    with vg.Gravity(len(x), 64, 4, "split", eps=eps) as g:
+        g.upload(x)
+        far_leaves = int(np.count_nonzero(g.leaf_inputs()["norm"] > 0))
+    assert cnt["fallback_leaves"] == far_leaves
+    assert np.median(_rel(af, a12)) < 1e-6 and _rel(af, a12).max() < 1e-4
+    b = cl.bound_for_fraction(params, io, X, 0.2)
+    ap, cntp = _forces(vg, x, 4, eps, params, io, bound=b)
+    assert 0.05 * far_leaves < cntp["fallback_leaves"] < 0.5 * far_leaves
+    e4, ep = _rel(a4, a12), _rel(ap, a12)
+    print(f"bound {b:.3e}: {cntp['fallback_leaves']} of {far_leaves} leaves fall back; median to p=12 {np.median(ep):.3e} (p=4 {np.median(e4):.3e})")
+    assert np.median(ep) < np.median(e4)

@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "lib", "libvipp_gravity.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-SOURCES = ["api.hip", "tree.hip", "sort.hip", "fmm.hip", "p2p.hip", "rk4.hip", "m2l_mfma.hip", "direct.hip"]
+SOURCES = ["api.hip", "tree.hip", "sort.hip", "fmm.hip", "p2p.hip", "rk4.hip", "m2l_mfma.hip", "direct.hip", "mlp.hip", "closure.hip"]
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wall", "-Wno-unused-function", "-munsafe-fp-atomics"]
 

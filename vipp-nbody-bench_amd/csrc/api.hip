@@ -5,6 +5,8 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <numeric>
+#include <random>
 
 #include "gravity.hpp"
 
@@ -16,6 +18,27 @@ namespace vg {
 
 void fmm_m2l_plain(Ctx &c);
 void fmm_m2l_split(Ctx &c);
+void fmm_m2l_plain_list(Ctx &c, const int32_t *parents, int np);
+void fmm_m2l_split_list(Ctx &c, const int32_t *parents, int np);
+
+void m2l_list(Ctx &c, const int32_t *parents, int np) {
+    if (c.prec == 2)
+        fmm_m2l_split_list(c, parents, np);
+    else
+        fmm_m2l_plain_list(c, parents, np);
+}
+
+void swap_expansions(Ctx &c, ExpSet &e) {
+    std::swap(c.p, e.p);
+    std::swap(c.K, e.K);
+    std::swap(c.ldk, e.ldk);
+    c.harm.swap(e.harm);
+    c.op64.swap(e.op64);
+    c.op32.swap(e.op32);
+    c.M32.swap(e.M32);
+    c.L32.swap(e.L32);
+    c.opsplit.swap(e.opsplit);
+}
 void fmm_constants_init();
 
 void phase_begin(Ctx &c, Phase p) {
@@ -157,7 +180,129 @@ int64_t ni_full_len(const HostLists &h, int32_t g) {
 }
 }  // namespace
 
+namespace {
+struct TmpStream {
+    hipStream_t s = nullptr;
+    explicit TmpStream(int device) {
+        VG_HIP(hipSetDevice(device));
+        VG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    }
+    ~TmpStream() {
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+}  // namespace
+
 extern "C" {
+
+int vipp_mlp_forward(int32_t device, const float *params, const float *x, int64_t n, float *y) {
+    return guarded(nullptr, [&] {
+        need(params && (n == 0 || (x && y)) && n >= 0, VIPP_GRAV_E_INVALID, "null argument");
+        TmpStream st(device);
+        vg::mlp::Net net;
+        net.par.ensure(vg::mlp::NPAR);
+        VG_HIP(hipMemcpyAsync(net.par.p, params, vg::mlp::NPAR * sizeof(float), hipMemcpyHostToDevice, st.s));
+        net.set_params(st.s, net.par);
+        if (n > 0) {
+            vg::DevBuf<float> xd, yd;
+            xd.ensure(n * vg::mlp::IN);
+            yd.ensure(n * vg::mlp::OUT);
+            VG_HIP(hipMemcpyAsync(xd.p, x, n * vg::mlp::IN * sizeof(float), hipMemcpyHostToDevice, st.s));
+            net.forward(st.s, xd, n, yd);
+            VG_HIP(hipMemcpyAsync(y, yd.p, n * vg::mlp::OUT * sizeof(float), hipMemcpyDeviceToHost, st.s));
+            VG_HIP(hipStreamSynchronize(st.s));
+        }
+        VG_HIP(hipStreamSynchronize(st.s));
+    });
+}
+
+int vipp_mlp_loss_grad(int32_t device, const float *params, const float *x, const float *t, const float *w, int64_t n, float *loss,
+                       float *grad) {
+    return guarded(nullptr, [&] {
+        need(params && x && t && loss && grad && n > 0, VIPP_GRAV_E_INVALID, "null argument or n <= 0");
+        TmpStream st(device);
+        vg::mlp::Net net;
+        net.par.ensure(vg::mlp::NPAR);
+        VG_HIP(hipMemcpyAsync(net.par.p, params, vg::mlp::NPAR * sizeof(float), hipMemcpyHostToDevice, st.s));
+        net.set_params(st.s, net.par);
+        vg::DevBuf<float> xd, td, wd, gd;
+        xd.ensure(n * vg::mlp::IN);
+        td.ensure(n * vg::mlp::OUT);
+        gd.ensure(vg::mlp::NPAR);
+        VG_HIP(hipMemcpyAsync(xd.p, x, n * vg::mlp::IN * sizeof(float), hipMemcpyHostToDevice, st.s));
+        VG_HIP(hipMemcpyAsync(td.p, t, n * vg::mlp::OUT * sizeof(float), hipMemcpyHostToDevice, st.s));
+        double wsum = 0;
+        if (w) {
+            for (int64_t i = 0; i < n; ++i) {
+                need(std::isfinite(w[i]) && w[i] >= 0, VIPP_GRAV_E_INVALID, "sample weights must be finite and >= 0");
+                wsum += w[i];
+            }
+            need(wsum > 0, VIPP_GRAV_E_INVALID, "sample weights sum to 0");
+            wd.ensure(n);
+            VG_HIP(hipMemcpyAsync(wd.p, w, n * sizeof(float), hipMemcpyHostToDevice, st.s));
+        }
+        *loss = net.loss_grad(st.s, xd, td, w ? wd.p : nullptr, wsum, n, gd);
+        VG_HIP(hipMemcpyAsync(grad, gd.p, vg::mlp::NPAR * sizeof(float), hipMemcpyDeviceToHost, st.s));
+        VG_HIP(hipStreamSynchronize(st.s));
+    });
+}
+
+int vipp_mlp_train(int32_t device, float *params, const float *x, const float *t, const float *w, int64_t n, uint32_t epochs, uint32_t batch,
+                   float lr, uint64_t seed, float *epoch_loss) {
+    return guarded(nullptr, [&] {
+        need(params && x && t && n > 0 && batch > 0 && epochs > 0, VIPP_GRAV_E_INVALID, "null argument or empty training set");
+        need(std::isfinite(lr) && lr > 0, VIPP_GRAV_E_INVALID, "learning rate must be positive");
+        TmpStream st(device);
+        using namespace vg::mlp;
+        Net net;
+        net.par.ensure(NPAR);
+        VG_HIP(hipMemcpyAsync(net.par.p, params, NPAR * sizeof(float), hipMemcpyHostToDevice, st.s));
+        net.set_params(st.s, net.par);
+        vg::DevBuf<float> xd, td, wd, bx, bt, bw, gd;
+        vg::DevBuf<uint32_t> idx;
+        xd.ensure(n * IN), td.ensure(n * OUT), idx.ensure(n), gd.ensure(NPAR);
+        const int64_t B = std::min<int64_t>(batch, n);
+        bx.ensure(B * IN), bt.ensure(B * OUT);
+        VG_HIP(hipMemcpyAsync(xd.p, x, n * IN * sizeof(float), hipMemcpyHostToDevice, st.s));
+        VG_HIP(hipMemcpyAsync(td.p, t, n * OUT * sizeof(float), hipMemcpyHostToDevice, st.s));
+        if (w) {
+            for (int64_t i = 0; i < n; ++i) need(std::isfinite(w[i]) && w[i] >= 0, VIPP_GRAV_E_INVALID, "sample weights must be finite and >= 0");
+            wd.ensure(n), bw.ensure(B);
+            VG_HIP(hipMemcpyAsync(wd.p, w, n * sizeof(float), hipMemcpyHostToDevice, st.s));
+        }
+        std::vector<uint32_t> perm(static_cast<size_t>(n));
+        int step = 0;
+        for (uint32_t e = 0; e < epochs; ++e) {
+            std::iota(perm.begin(), perm.end(), 0u);
+            std::mt19937_64 rng(seed + e);
+            for (int64_t i = n - 1; i > 0; --i) {  // Fisher-Yates with a fixed generator: reproducible order
+                const int64_t j = static_cast<int64_t>(rng() % static_cast<uint64_t>(i + 1));
+                std::swap(perm[static_cast<size_t>(i)], perm[static_cast<size_t>(j)]);
+            }
+            VG_HIP(hipMemcpyAsync(idx.p, perm.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, st.s));
+            const float lr_e = lr * (0.05f + 0.95f * 0.5f * (1.0f + std::cos(3.14159265358979f * static_cast<float>(e) / static_cast<float>(epochs))));
+            double sum = 0, wall = 0;
+            for (int64_t off = 0; off < n; off += B) {
+                const int64_t nb = std::min<int64_t>(B, n - off);
+                gather_rows(st.s, xd, IN, idx.p + off, nb, bx);
+                gather_rows(st.s, td, OUT, idx.p + off, nb, bt);
+                double wsum = static_cast<double>(nb);
+                if (w) {
+                    gather_rows(st.s, wd, 1, idx.p + off, nb, bw);
+                    wsum = 0;
+                    for (int64_t i = 0; i < nb; ++i) wsum += w[perm[static_cast<size_t>(off + i)]];
+                    if (!(wsum > 0)) continue;  // a batch of zero-weight rows carries no gradient
+                }
+                sum += static_cast<double>(net.loss_grad(st.s, bx, bt, w ? bw.p : nullptr, wsum, nb, gd)) * wsum;
+                wall += wsum;
+                net.adam_step(st.s, gd, lr_e, ++step);
+            }
+            if (epoch_loss) epoch_loss[e] = static_cast<float>(wall > 0 ? sum / wall : 0.0);
+        }
+        VG_HIP(hipMemcpyAsync(params, net.par.p, NPAR * sizeof(float), hipMemcpyDeviceToHost, st.s));
+        VG_HIP(hipStreamSynchronize(st.s));
+    });
+}
 
 int vipp_grav_create(const vipp_grav_config *cfg, vipp_grav_ctx **out) {
     return guarded(nullptr, [&] {
@@ -328,6 +473,59 @@ int vipp_grav_direct(vipp_grav_ctx *ctx, const int64_t *idx, int64_t nidx, doubl
         need(c.uploaded, VIPP_GRAV_E_STATE, "direct before upload");
         VG_HIP(hipSetDevice(c.device));
         vg::direct_subset(c, idx, nidx, a, pot);
+    });
+}
+
+int vipp_grav_leaf_inputs(vipp_grav_ctx *ctx, int64_t *n, float *x, float *norm, float *l25, int32_t *level) {
+    return guarded(ctx, [&] {
+        need(ctx != nullptr && n != nullptr, VIPP_GRAV_E_INVALID, "null argument");
+        vg::Ctx &c = ctx->c;
+        need(c.prec != VIPP_GRAV_F64, VIPP_GRAV_E_INVALID, "leaf inputs need FP32 expansions (precision f32 or split)");
+        need(c.p >= 4, VIPP_GRAV_E_INVALID, "leaf inputs need p >= 4 (25 coefficients)");
+        need(c.uploaded, VIPP_GRAV_E_STATE, "no particles uploaded");
+        VG_HIP(hipSetDevice(c.device));
+        if (!c.evaluated) vg::evaluate(c, false);
+        const int64_t nl = c.t.nleaves;
+        *n = nl;
+        if (!x && !norm && !l25 && !level) return;
+        vg::closure_inputs(c, true);
+        if (nl) {
+            if (x) VG_HIP(hipMemcpyAsync(x, c.cl.x.p, nl * 28 * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+            if (norm) VG_HIP(hipMemcpyAsync(norm, c.cl.norm.p, nl * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+            if (l25) VG_HIP(hipMemcpyAsync(l25, c.cl.l25.p, nl * 25 * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+            if (level) VG_HIP(hipMemcpyAsync(level, c.cl.lev.p, nl * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+        }
+        VG_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int vipp_grav_set_closure(vipp_grav_ctx *ctx, const float *params, const float *io, double bound, uint32_t p_fallback) {
+    return guarded(ctx, [&] {
+        need(ctx != nullptr, VIPP_GRAV_E_INVALID, "null context");
+        vg::Ctx &c = ctx->c;
+        VG_HIP(hipSetDevice(c.device));
+        if (!params) {
+            c.cl.on = false;
+            c.evaluated = false;
+            return;
+        }
+        need(c.prec != VIPP_GRAV_F64, VIPP_GRAV_E_INVALID, "the closure needs FP32 expansions (precision f32 or split)");
+        need(c.p >= 4, VIPP_GRAV_E_INVALID, "the closure needs p >= 4 (it corrects 25 coefficients)");
+        need(!(bound < 0), VIPP_GRAV_E_INVALID, "bound must be >= 0 (use INFINITY for no fallback)");
+        need(p_fallback > static_cast<uint32_t>(c.p) && p_fallback <= static_cast<uint32_t>(vg::MAXP), VIPP_GRAV_E_INVALID,
+             "fallback order must be in (p, 12]");
+        c.cl.net.par.ensure(vg::mlp::NPAR);
+        VG_HIP(hipMemcpyAsync(c.cl.net.par.p, params, vg::mlp::NPAR * sizeof(float), hipMemcpyHostToDevice, c.stream));
+        c.cl.net.set_params(c.stream, c.cl.net.par);
+        std::vector<float> iov(81);
+        for (int k = 0; k < 81; ++k) iov[static_cast<size_t>(k)] = io ? io[k] : (k < 28 ? 0.0f : 1.0f);
+        c.cl.io.ensure(81);
+        VG_HIP(hipMemcpyAsync(c.cl.io.p, iov.data(), 81 * sizeof(float), hipMemcpyHostToDevice, c.stream));
+        VG_HIP(hipStreamSynchronize(c.stream));
+        c.cl.on = true;
+        c.cl.bound = bound;
+        c.cl.p_hi = static_cast<int>(p_fallback);
+        c.evaluated = false;
     });
 }
 
@@ -506,6 +704,7 @@ int vipp_grav_counts(vipp_grav_ctx *ctx, double *out, int32_t n) {
         need(c.uploaded, VIPP_GRAV_E_STATE, "no particles uploaded");
         if (!c.evaluated) vg::evaluate(c, false);
         vg::count_work(c);
+        c.counts[6] = static_cast<double>(c.cl.on ? c.cl.nflag : 0);
         for (int i = 0; i < n && i < 8; ++i) out[i] = c.counts[i];
     });
 }

@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace vg {
@@ -60,6 +61,10 @@ struct DevBuf {
         }
         p = q;
         cap = nc;
+    }
+    void swap(DevBuf &o) noexcept {
+        std::swap(p, o.p);
+        std::swap(cap, o.cap);
     }
     operator T *() const { return p; }
 };

@@ -417,8 +417,9 @@ template <class R>
 __global__ __launch_bounds__(64) void k_l2p(const int32_t *__restrict__ leaves, const int8_t *__restrict__ level, const uint32_t *__restrict__ start,
                                             const uint32_t *__restrict__ count, const double *__restrict__ bmin, const double *__restrict__ bmax,
                                             const double *__restrict__ xs64, double wx0, int p, int K, int ldk, const R *__restrict__ Lx,
-                                            double gscale, R *__restrict__ acc4) {
+                                            double gscale, R *__restrict__ acc4, const uint8_t *__restrict__ skip) {
     __shared__ R sl[MAXK];
+    if (skip && skip[blockIdx.x]) return;  // the leaf takes the closure's fallback instead
     const int32_t c = leaves[blockIdx.x];
     if (level[c] < 2) return;  // solver_ref.hpp:220-221
     for (int i = threadIdx.x; i < K; i += 64) sl[i] = Lx[static_cast<size_t>(c) * ldk + i];
@@ -526,16 +527,33 @@ void upward_t(Ctx &c) {
 }
 
 template <class R>
+void m2l_plain_list_t(Ctx &c, const int32_t *parents, int np) {
+    const TreeDev &t = c.t;
+    if (np <= 0) return;
+    constexpr int TB = sizeof(R) == 4 ? 32 : 16;
+    dim3 grid(ceil_div(np, TB), 8);
+    k_m2l_plain<R, TB><<<grid, 256, 0, c.stream>>>(parents, np, t.child, t.count, t.nbr27, Bufs<R>::M(c), Bufs<R>::L(c), Bufs<R>::op(c), c.K, c.ldk);
+    VG_LAUNCH_CHECK();
+}
+
+template <class R>
 void m2l_plain_t(Ctx &c) {
     TreeDev &t = c.t;
     // parents: internal cells of levels >= 1 (their children are the targets at level >= 2)
     if (t.depth < 1) return;
     const int64_t a = t.int_off[1], b = t.int_off.back();
     if (b <= a) return;
-    constexpr int TB = sizeof(R) == 4 ? 32 : 16;
-    dim3 grid(ceil_div(b - a, TB), 8);
-    k_m2l_plain<R, TB><<<grid, 256, 0, c.stream>>>(t.int_ids.p + a, static_cast<int>(b - a), t.child, t.count, t.nbr27, Bufs<R>::M(c), Bufs<R>::L(c),
-                                                   Bufs<R>::op(c), c.K, c.ldk);
+    m2l_plain_list_t<R>(c, t.int_ids.p + a, static_cast<int>(b - a));
+}
+
+template <class R>
+void l2p_t(Ctx &c, const int32_t *leaves, int64_t n, const uint8_t *skip) {
+    const TreeDev &t = c.t;
+    const double wx0 = t.box[3] - t.box[0];
+    const double gscale = (c.flags & 1u) ? 1.0 : c.G;
+    if (n)
+        k_l2p<R><<<static_cast<unsigned>(n), 64, 0, c.stream>>>(leaves, t.level, t.start, t.count, t.bmin, t.bmax, c.xs64, wx0, c.p, c.K, c.ldk,
+                                                                Bufs<R>::L(c), gscale, Bufs<R>::acc(c), skip);
     VG_LAUNCH_CHECK();
 }
 
@@ -546,12 +564,17 @@ void downward_t(Ctx &c) {
         const int64_t a = t.int_off[d], b = t.int_off[d + 1];
         if (b > a) k_l2l<R><<<static_cast<unsigned>(b - a), 256, 0, c.stream>>>(t.int_ids.p + a, t.child, t.count, Bufs<R>::op(c), c.K, c.ldk, Bufs<R>::L(c));
     }
-    const double wx0 = t.box[3] - t.box[0];
-    const double gscale = (c.flags & 1u) ? 1.0 : c.G;
-    if (t.nleaves)
-        k_l2p<R><<<static_cast<unsigned>(t.nleaves), 64, 0, c.stream>>>(t.leaf_ids, t.level, t.start, t.count, t.bmin, t.bmax, c.xs64, wx0, c.p, c.K,
-                                                                       c.ldk, Bufs<R>::L(c), gscale, Bufs<R>::acc(c));
-    VG_LAUNCH_CHECK();
+    bool fallback = false;
+    if constexpr (sizeof(R) == 4) {
+        if (c.cl.on) {
+            closure_apply(c);                         // learned correction of the leaf locals before L2P
+            fallback = closure_fallback_prepare(c);   // order-p_hi locals where the estimate exceeds the bound
+        }
+    }
+    l2p_t<R>(c, t.leaf_ids, t.nleaves, fallback ? c.cl.flag.p : nullptr);
+    if constexpr (sizeof(R) == 4) {
+        if (fallback) closure_fallback_l2p(c);
+    }
 }
 
 }  // namespace
@@ -655,6 +678,20 @@ void fmm_m2l_plain(Ctx &c) {
     else
         m2l_plain_t<float>(c);
 }
+
+void fmm_m2l_plain_list(Ctx &c, const int32_t *parents, int np) {
+    if (c.prec == 0)
+        m2l_plain_list_t<double>(c, parents, np);
+    else
+        m2l_plain_list_t<float>(c, parents, np);
+}
+
+void l2l_list(Ctx &c, const int32_t *parents, int64_t np) {
+    if (np > 0) k_l2l<float><<<static_cast<unsigned>(np), 256, 0, c.stream>>>(parents, c.t.child, c.t.count, c.op32, c.K, c.ldk, c.L32);
+    VG_LAUNCH_CHECK();
+}
+
+void l2p_list(Ctx &c, const int32_t *leaves, int64_t n, const uint8_t *skip) { l2p_t<float>(c, leaves, n, skip); }
 
 void fmm_downward(Ctx &c, bool) {
     if (c.prec == 0)

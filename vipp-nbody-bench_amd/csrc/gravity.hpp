@@ -7,6 +7,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "mlp.hpp"
 
 namespace vg {
 
@@ -41,6 +42,37 @@ struct TreeDev {
     // reference numbering (built on demand for export)
     DevBuf<int32_t> ref_id;
     bool ref_valid = false;
+};
+
+// the expansion order and everything sized by it; the closure's fallback keeps
+// a second set (order p_hi) and swaps it into the context while it runs
+struct ExpSet {
+    int p = 0, K = 0, ldk = 0;
+    DevBuf<double> harm, op64;
+    DevBuf<float> op32, M32, L32;
+    DevBuf<uint16_t> opsplit;
+};
+
+// learned closure state (closure.hip)
+struct Closure {
+    bool on = false;
+    double bound = 0;            // fallback where |MLP(x)| exceeds it
+    int p_hi = 12;               // order of the fallback far field
+    mlp::Net net;
+    DevBuf<float> x, norm, y;    // per non-empty leaf: network input, |L~_25|, output
+    DevBuf<float> l25;           // raw first 25 coefficients (export)
+    DevBuf<int32_t> lev;         // leaf level (export)
+    DevBuf<uint8_t> flag;        // per leaf: take the fallback
+    DevBuf<uint32_t> scal;       // flagged-leaf counter
+    int64_t nflag = 0;
+    DevBuf<float> io;            // input shift (28), input scale (28), output scale (25)
+    // fallback
+    ExpSet hi;                   // order-p_hi expansions and operators
+    DevBuf<uint8_t> need;        // per cell: on the ancestor chain of a flagged leaf
+    DevBuf<int32_t> fb_leaves;   // flagged leaves
+    DevBuf<int32_t> fb_int;      // needed internal cells, level by level
+    DevBuf<uint32_t> fb_pos;     // compaction scratch
+    std::vector<int64_t> fb_off; // fb_int offset of each level (depth + 2 entries)
 };
 
 struct Ctx {
@@ -110,6 +142,8 @@ struct Ctx {
     DevBuf<double> p2p_part64;
 
     double counts[8] = {};
+
+    Closure cl;
 };
 
 // phases (each launches on ctx.stream)
@@ -127,6 +161,14 @@ void rk4_stage(Ctx &c, int stage, double dt);
 void count_work(Ctx &c);
 double kinetic_energy(Ctx &c);
 double potential_energy(Ctx &c);
+void closure_inputs(Ctx &c, bool with_l25);  // network inputs of every non-empty leaf from the current L32
+void closure_apply(Ctx &c);                   // L~_25 += |L~_25| MLP(x) and fallback flags
+bool closure_fallback_prepare(Ctx &c);        // order-p_hi locals of the flagged leaves; false if none is flagged
+void closure_fallback_l2p(Ctx &c);            // order-p_hi L2P of the flagged leaves
+void swap_expansions(Ctx &c, ExpSet &e);      // exchange the context's expansion set with e
+void m2l_list(Ctx &c, const int32_t *parents, int np);      // M2L of the children of the given internal cells
+void l2l_list(Ctx &c, const int32_t *parents, int64_t np);  // L2L into the children of the given internal cells (FP32)
+void l2p_list(Ctx &c, const int32_t *leaves, int64_t n, const uint8_t *skip);  // L2P of the given leaves (FP32)
 void direct_subset(Ctx &c, const int64_t *idx, int64_t nidx, double *a, double *pot);  // FP64 O(N nidx) oracle forces
 
 void phase_begin(Ctx &c, Phase p);

@@ -275,13 +275,8 @@ void m2l_split_operators(Ctx &c, int nvec) {
     VG_LAUNCH_CHECK();
 }
 
-void fmm_m2l_split(Ctx &c) {
-    TreeDev &t = c.t;
-    if (t.depth < 1) return;
-    const int64_t a = t.int_off[1], b = t.int_off.back();
-    if (b <= a) return;
-    const int32_t *parents = t.int_ids.p + a;
-    const int np = static_cast<int>(b - a);
+void fmm_m2l_split_list(Ctx &c, const int32_t *parents, int np) {
+    if (np <= 0) return;
     switch (c.p) {
         case 1: launch_split<1>(c, parents, np); break;
         case 2: launch_split<2>(c, parents, np); break;
@@ -297,6 +292,14 @@ void fmm_m2l_split(Ctx &c) {
         case 12: launch_split<12>(c, parents, np); break;
         default: throw Error(1, "split M2L supports 1 <= p <= 12");
     }
+}
+
+void fmm_m2l_split(Ctx &c) {
+    TreeDev &t = c.t;
+    if (t.depth < 1) return;
+    const int64_t a = t.int_off[1], b = t.int_off.back();
+    if (b <= a) return;
+    fmm_m2l_split_list(c, t.int_ids.p + a, static_cast<int>(b - a));
 }
 
 }  // namespace vg

@@ -22,6 +22,7 @@ FLAG_REF_FAR_G = 1
 PHASES = ["tree", "lists", "upward", "m2l", "p2p", "downward", "rk4", "operators"]
 
 E_INVALID, E_INPUT, E_DEPTH, E_DEVICE, E_STATE = 1, 2, 3, 4, 5
+MLP_IN, MLP_HIDDEN, MLP_OUT, MLP_NPARAMS = 28, 128, 25, 23449
 
 
 class GravityError(RuntimeError):
@@ -70,6 +71,11 @@ def _declare(lib):
         "vipp_grav_set_timing": [vp, i32],
         "vipp_grav_device_state": [vp, P(vp), P(vp)],
         "vipp_grav_direct": [vp, vp, i64, vp, vp],
+        "vipp_grav_leaf_inputs": [vp, P(i64), vp, vp, vp, vp],
+        "vipp_grav_set_closure": [vp, vp, vp, d, C.c_uint32],
+        "vipp_mlp_forward": [i32, vp, vp, i64, vp],
+        "vipp_mlp_loss_grad": [i32, vp, vp, vp, vp, i64, P(C.c_float), vp],
+        "vipp_mlp_train": [i32, vp, vp, vp, vp, i64, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, vp],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -181,6 +187,30 @@ class Gravity:
         self._check(self.lib.vipp_grav_direct(self.h, _ptr(idx), len(idx), _ptr(a), _ptr(pot)))
         return (a.reshape(-1, 3), pot) if potential else a.reshape(-1, 3)
 
+    def leaf_inputs(self):
+        """Closure view of the last evaluation: dict of x (n x 28), norm, l25 (n x 25), level."""
+        n = C.c_int64()
+        self._check(self.lib.vipp_grav_leaf_inputs(self.h, C.byref(n), None, None, None, None))
+        k = n.value
+        x = np.zeros((max(1, k), 28), np.float32)
+        norm = np.zeros(max(1, k), np.float32)
+        l25 = np.zeros((max(1, k), 25), np.float32)
+        lev = np.zeros(max(1, k), np.int32)
+        self._check(self.lib.vipp_grav_leaf_inputs(self.h, C.byref(n), _ptr(x), _ptr(norm), _ptr(l25), _ptr(lev)))
+        return {"x": x[:k], "norm": norm[:k], "l25": l25[:k], "level": lev[:k]}
+
+    def set_closure(self, params, io=None, bound=float("inf"), p_fallback=12):
+        """Enable the learned closure (params: VIPP_MLP_NPARAMS float32; io: 81
+        float32 standardisation values or None) or disable it (params None)."""
+        if params is None:
+            self._check(self.lib.vipp_grav_set_closure(self.h, None, None, 0.0, 0))
+            return
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        assert params.size == MLP_NPARAMS
+        io = None if io is None else np.ascontiguousarray(io, dtype=np.float32)
+        assert io is None or io.size == 81
+        self._check(self.lib.vipp_grav_set_closure(self.h, _ptr(params), _ptr(io), float(bound), int(p_fallback)))
+
     def energy(self):
         ke, pe = C.c_double(), C.c_double()
         self._check(self.lib.vipp_grav_energy(self.h, C.byref(ke), C.byref(pe)))
@@ -225,7 +255,7 @@ class Gravity:
         out = np.zeros(8)
         self._check(self.lib.vipp_grav_counts(self.h, _ptr(out), 8))
         return {"p2p_pairs": out[0], "m2l_pairs": out[1], "cells": int(out[2]), "leaves": int(out[3]), "depth": int(out[4]),
-                "m2l_live_pairs": out[5]}
+                "m2l_live_pairs": out[5], "fallback_leaves": int(out[6])}
 
     def set_timing(self, on=True):
         self._check(self.lib.vipp_grav_set_timing(self.h, 1 if on else 0))
@@ -236,3 +266,43 @@ class Gravity:
         if reset:
             self._check(self.lib.vipp_grav_phase_times(self.h, None, 0))
         return dict(zip(PHASES, ms.tolist()))
+
+
+def _mlp_check(lib, rc):
+    if rc != 0:
+        raise GravityError(rc, lib.vipp_grav_last_error(None).decode())
+
+
+def mlp_forward(params, x, device=0):
+    """Network outputs (n x 25) on the device."""
+    lib = library()
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.zeros((len(x), MLP_OUT), np.float32)
+    _mlp_check(lib, lib.vipp_mlp_forward(device, _ptr(params), _ptr(x), len(x), _ptr(y)))
+    return y
+
+
+def _f32(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float32)
+
+
+def mlp_loss_grad(params, x, t, w=None, device=0):
+    """((weighted) mean squared error, gradient) on the device."""
+    lib = library()
+    params, x, t, w = _f32(params), _f32(x), _f32(t), _f32(w)
+    loss = C.c_float()
+    g = np.zeros(MLP_NPARAMS, np.float32)
+    _mlp_check(lib, lib.vipp_mlp_loss_grad(device, _ptr(params), _ptr(x), _ptr(t), _ptr(w), len(x), C.byref(loss), _ptr(g)))
+    return float(loss.value), g
+
+
+def mlp_train(params, x, t, epochs, batch, lr, seed=0, w=None, device=0):
+    """Adam training on the device; returns (new params, per-epoch mean loss)."""
+    lib = library()
+    params = np.array(params, dtype=np.float32, copy=True)
+    x, t, w = _f32(x), _f32(t), _f32(w)
+    hist = np.zeros(int(epochs), np.float32)
+    _mlp_check(lib, lib.vipp_mlp_train(device, _ptr(params), _ptr(x), _ptr(t), _ptr(w), len(x), int(epochs), int(batch), float(lr), int(seed),
+                                       _ptr(hist)))
+    return params, hist
