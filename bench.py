@@ -50,6 +50,11 @@ def main():
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this size, the known-size kernel that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to HBM bytes")
+    ap.add_argument("--learned-workload", default="mixture4m_p4",
+                    help="workload of the learned-closure comparison (p=4 + closure against plain FP32 p=12); 'none' skips it")
+    ap.add_argument("--fallback-fraction", type=float, default=0.1,
+                    help="fraction of leaves (on the training pairs) whose closure estimate sends them to the order-12 far field")
+    ap.add_argument("--learned-err-sample", type=int, default=16384)
     ap.add_argument("--no-compare-plain", dest="compare_plain", action="store_false",
                     help="skip the plain FP32 p=12 GPU FMM run reported beside the main mode")
     args = ap.parse_args()
@@ -132,6 +137,10 @@ def main():
     if args.compare_plain and mode != "f32":
         plain = plain_reference(vgm, x, v, masses, w, n, args.steps, value)
 
+    learned = None
+    if args.learned_workload != "none":
+        learned = learned_comparison(vgm, workloads, args)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(x, w, n)
@@ -146,6 +155,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "vs_plain_gpu": plain,
+        "learned": learned,
         "host_roundtrip_steps_per_s": pcie_steps,
         "phase_ms_per_eval": per_eval,
         "p2p_pairs_per_s": cnt["p2p_pairs"] / (per_eval["p2p"] * 1e-3) if per_eval["p2p"] > 0 else None,
@@ -170,6 +180,71 @@ def plain_reference(vgm, x, v, masses, w, n, steps, value):
     g.close()
     plain = steps / (t1 - t0)
     return {"mode": "plain fp32 p2p + fp32 m2l", "steps_per_s": plain, "speedup": value / plain}
+
+
+def learned_comparison(vgm, workloads, args):
+    """p = 4 FMM + learned closure (+ order-12 fallback) against the plain FP32
+    p = 12 GPU FMM on the same input: steps/s and force errors.  The network is
+    trained on the device on pairs the FMM generates from three other seeds
+    of the workload (never the evaluated one)."""
+    import closure
+    wl = args.learned_workload
+    wdef = workloads.WORKLOADS[wl]
+    s_, eps, dt = wdef["s"], wdef["eps"], wdef["dt"]
+    t0 = time.perf_counter()
+    X, T = [], []
+    for seed in (1, 2, 3):
+        xs, _, ms = workloads.make(wl, seed=seed)
+        a, b = closure.training_pairs(xs, s_, eps, ms)
+        X.append(a)
+        T.append(b)
+    X, T = np.concatenate(X), np.concatenate(T)
+    t1 = time.perf_counter()
+    params, io, hist = closure.train(X, T, epochs=40, batch=512, lr=2e-3, seed=0)
+    t2 = time.perf_counter()
+    bound = closure.bound_for_fraction(params, io, X, args.fallback_fraction)
+    x, v, masses = workloads.make(wl, seed=0)
+    n = len(x)
+    idx = np.sort(np.random.default_rng(1).choice(n, size=min(n, args.learned_err_sample), replace=False))
+
+    def run(p, mode, with_closure, b=float("inf")):
+        g = vgm.Gravity(n, s_, p, mode, eps=eps, G=1.0, mass=1.0 / n)
+        g.upload(x, v, masses)
+        if with_closure:
+            g.set_closure(params, io, b)
+        a = g.accelerations()
+        cnt = g.counts()
+        d = g.direct(idx) if run.d is None else run.d
+        run.d = d
+        g.step(dt, 1)  # warm-up
+        g.upload(x, v, masses)
+        g.set_timing(True)
+        g.phase_times(reset=True)
+        ta = time.perf_counter()
+        g.step(dt, args.steps)
+        tb = time.perf_counter()
+        ph = g.phase_times()
+        g.close()
+        e = np.linalg.norm(a[idx] - d, axis=1) / np.linalg.norm(d, axis=1)
+        return {"steps_per_s": args.steps / (tb - ta), "err_median": float(np.median(e)), "err_p99": float(np.percentile(e, 99)),
+                "phase_ms_per_eval": {k: v_ / (4 * args.steps) for k, v_ in ph.items()}, "fallback_leaves": cnt["fallback_leaves"],
+                "leaves": cnt["leaves"]}, a
+    run.d = None
+    plain, a12 = run(12, "f32", False)
+    split12, _ = run(12, "split", False)
+    low, a4 = run(4, "split", False)
+    learned, ac = run(4, "split", True, bound)
+    learned_nofb, acn = run(4, "split", True)
+    judge = lambda a: float(np.median(np.linalg.norm(a[idx] - a12[idx], axis=1) / np.linalg.norm(a12[idx], axis=1)))
+    return {
+        "workload": wl, "n": n, "train_pairs": int(len(X)), "train_seeds": [1, 2, 3], "eval_seed": 0,
+        "pairs_s": t1 - t0, "train_s": t2 - t1, "train_loss_first_last": [float(hist[0]), float(hist[-1])],
+        "fallback_fraction_target": args.fallback_fraction, "bound": bound,
+        "plain_fp32_p12": plain, "split_p12": split12, "p4_no_closure": low, "p4_closure": learned_nofb, "p4_closure_fallback": learned,
+        "speedup_vs_plain": learned["steps_per_s"] / plain["steps_per_s"],
+        "median_diff_to_p12": {"p4": judge(a4), "p4_closure": judge(acn), "p4_closure_fallback": judge(ac)},
+        "err_sample": int(len(idx)), "err_reference": "fp64 direct sum on the device",
+    }
 
 
 def calibration_copy(nbytes):
