@@ -58,63 +58,53 @@ __device__ inline dc Yget(const double *tab, int l, int mu) {
 
 constexpr int HSTRIDE = 2 * ((AL + 1) * (AL + 2) / 2) + 2;  // per offset: Y table + rho
 
-// One thread per offset vector: Y_l^m(direction) for l <= lmax and |v|.
-__global__ void k_harmonics(const double *__restrict__ vec, int nvec, int lmax, double *__restrict__ out) {
-    const int o = blockIdx.x * blockDim.x + threadIdx.x;
-    if (o >= nvec) return;
+// Y_l^m(direction) for l <= lmax and |v| of each offset vector.
+// One thread per (offset, m): P_l^m by the upward recurrence in l, with
+// P_m^m = (-1)^m (2m-1)!! sin^m, and sqrt((l-m)!/(l+m)!) carried along l.
+__global__ __launch_bounds__(64) void k_harmonics(const double *__restrict__ vec, int nvec, int lmax, double *__restrict__ out) {
+    const int o = blockIdx.x;
+    const int m = threadIdx.x;
+    if (o >= nvec || m > lmax) return;
     const double vx = vec[3 * o], vy = vec[3 * o + 1], vz = vec[3 * o + 2];
     double *tab = out + static_cast<size_t>(o) * HSTRIDE;
     const double rho = sqrt(vx * vx + vy * vy + vz * vz);
-    tab[HSTRIDE - 2] = rho;
-    tab[HSTRIDE - 1] = 0;
+    if (m == 0) {
+        tab[HSTRIDE - 2] = rho;
+        tab[HSTRIDE - 1] = 0;
+    }
     const double rxy = sqrt(vx * vx + vy * vy);
     const double ct = rho > 0 ? vz / rho : 1.0, st = rho > 0 ? rxy / rho : 0.0;
     const dc eph = rxy > 0 ? dc{vx / rxy, vy / rxy} : dc{1.0, 0.0};
     dc em{1.0, 0.0};  // e^{i m phi}
-    double pmm = 1.0;
-    for (int m = 0; m <= lmax; ++m) {
-        if (m > 0) {
-            pmm *= -(2.0 * m - 1) * st;
-            em = cmul(em, eph);
+    double pmm = 1.0, f = 1.0;  // P_m^m, (l-m)!/(l+m)! at l = m: 1/(2m)!
+    for (int i = 1; i <= m; ++i) {
+        pmm *= -(2.0 * i - 1) * st;
+        em = cmul(em, eph);
+        f /= (2.0 * i - 1) * (2.0 * i);
+    }
+    double p0 = 0, p1 = pmm;
+    for (int l = m; l <= lmax; ++l) {
+        double pl;
+        if (l == m)
+            pl = pmm;
+        else if (l == m + 1)
+            pl = ct * (2.0 * m + 1) * pmm;
+        else
+            pl = ((2.0 * l - 1) * ct * p1 - (l + m - 1.0) * p0) / (l - m);
+        if (l > m) {
+            p0 = p1;
+            p1 = pl;
+            f *= static_cast<double>(l - m) / static_cast<double>(l + m);
         }
-        double p0 = 0, p1 = pmm;
-        for (int l = m; l <= lmax; ++l) {
-            double pl;
-            if (l == m)
-                pl = pmm;
-            else if (l == m + 1)
-                pl = ct * (2.0 * m + 1) * pmm;
-            else
-                pl = ((2.0 * l - 1) * ct * p1 - (l + m - 1.0) * p0) / (l - m);
-            if (l > m) {
-                p0 = p1;
-                p1 = pl;
-            }
-            double f = 1.0;
-            for (int i = l - m + 1; i <= l + m; ++i) f /= i;
-            const double nrm = sqrt(f) * pl;
-            tab[2 * (l * (l + 1) / 2 + m)] = nrm * em.re;
-            tab[2 * (l * (l + 1) / 2 + m) + 1] = nrm * em.im;
-        }
+        const double nrm = sqrt(f) * pl;
+        tab[2 * (l * (l + 1) / 2 + m)] = nrm * em.re;
+        tab[2 * (l * (l + 1) / 2 + m) + 1] = nrm * em.im;
     }
 }
 
-// Fill one real operator row from complex coefficients T(+m) and T(-m).
-// row part 0 = Re, 1 = Im; column (n, m) -> Re at kidx_re, Im at kidx_im.
-struct RowSink {
-    double *row;
-    int part;
-    __device__ void put(int n, int m, dc tp, dc tm) {
-        if (m == 0) {
-            row[kidx_re(n, 0)] += part == 0 ? tp.re : tp.im;
-        } else {
-            const dc s = cadd(tp, tm), d = csub(tp, tm);
-            row[kidx_re(n, m)] += part == 0 ? s.re : s.im;
-            row[kidx_im(n, m)] += part == 0 ? -d.im : d.re;
-        }
-    }
-};
-
+// Row r of an operator in the real-packed basis: degree j, order k, part
+// 0 = Re (cosine) or 1 = Im (sine).  Column pair (n, m): Re at kidx_re, Im at
+// kidx_im, from the complex coefficients T(+m), T(-m).
 __device__ inline void row_of(int r, int &j, int &k, int &part) {
     j = static_cast<int>(sqrt(static_cast<double>(r)));
     while (j * j > r) --j;
@@ -129,192 +119,247 @@ __device__ inline void row_of(int r, int &j, int &k, int &part) {
     }
 }
 
-// op block layout: [M2M 0..7][L2L 8..15][M2L 16..16+342], each K*K row-major [row][col].
+// op block layout: [M2M 0..7][L2L 8..15][M2L 16..16+342], each K*K; M2L
+// blocks row-major [row][col], M2M and L2L blocks column-major [col][row] (their
+// kernels give each lane one output row, so lanes then read consecutive words).
+// One block per (operator, row r); thread t takes the column pair (n, m >= 0)
+// number t, i.e. the real-packed columns of degree n, order m.
 template <class R>
-__global__ __launch_bounds__(64) void k_ops(const double *__restrict__ harm, int p, int K, R *__restrict__ op) {
+__global__ __launch_bounds__(128) void k_ops(const double *__restrict__ harm, int p, int K, R *__restrict__ op) {
     const int blk = blockIdx.x;  // operator index
-    const int r = blockIdx.y * 64 + threadIdx.x;
-    if (r >= K) return;
+    const int r = blockIdx.y;
     int j, k, part;
     row_of(r, j, k, part);
-    double row[MAXK];
-    for (int c = 0; c < K; ++c) row[c] = 0;
-    RowSink sink{row, part};
+    const bool cmaj = blk < 16;
+    R *dst = cmaj ? op + static_cast<size_t>(blk) * K * K + r : op + (static_cast<size_t>(blk) * K + r) * K;
+    const size_t cs = cmaj ? static_cast<size_t>(K) : 1;  // column stride
+    for (int c = threadIdx.x; c < K; c += 128) dst[c * cs] = R(0);
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t >= (p + 1) * (p + 2) / 2) return;
+    int n = 0;
+    while ((n + 1) * (n + 2) / 2 <= t) ++n;
+    const int mp = t - n * (n + 1) / 2;  // column pair (n, mp)
     const double *tab = harm + static_cast<size_t>(blk) * HSTRIDE;
     const double rho = tab[HSTRIDE - 2];
+    dc tpm[2] = {{0, 0}, {0, 0}};
     if (blk < 8) {
-        // M2M, child offset q: column (j', k') of the child expansion, scaled by 2^{-j'} (fmm.hpp:256-270)
-        for (int jp = 0; jp <= j; ++jp) {
-            const int n = j - jp;
-            const double rn = pow(rho, n);
-            const double scale = ldexp(1.0, -jp);
-            for (int kp = 0; kp <= jp; ++kp) {
-                dc tpm[2] = {{0, 0}, {0, 0}};
-                for (int sgn = 0; sgn < 2; ++sgn) {
-                    const int kk = sgn == 0 ? kp : -kp;
-                    const int m = k - kk;
-                    if (m < -n || m > n) continue;
-                    const double coef = ipow_even(abs(k) - abs(m) - abs(k - m)) * Acoef(n, m) * Acoef(jp, kk) * rn / Acoef(j, k) * scale;
-                    tpm[sgn] = cscale(Yget(tab, n, -m), coef);
-                }
-                sink.put(jp, kp, tpm[0], tpm[1]);
-            }
+        // M2M, child offset q: column (j', k') = (n, mp) of the child expansion, scaled by 2^{-j'} (fmm.hpp:256-270)
+        const int jp = n, kp = mp;
+        if (jp > j) return;
+        const int nn = j - jp;
+        const double rn = pow(rho, nn);
+        const double scale = ldexp(1.0, -jp);
+        for (int sgn = 0; sgn < 2; ++sgn) {
+            const int kk = sgn == 0 ? kp : -kp;
+            const int m = k - kk;
+            if (m < -nn || m > nn) continue;
+            const double coef = ipow_even(abs(k) - abs(m) - abs(k - m)) * Acoef(nn, m) * Acoef(jp, kk) * rn / Acoef(j, k) * scale;
+            tpm[sgn] = cscale(Yget(tab, nn, -m), coef);
         }
     } else if (blk < 16) {
         // L2L, child offset q: column (n, m) of the parent expansion, scaled by 2^{-(j+1)} (fmm.hpp:313-327)
+        if (n < j) return;
         const double scale = ldexp(1.0, -(j + 1));
-        for (int n = j; n <= p; ++n) {
-            const double rn = pow(rho, n - j);
-            for (int mp = 0; mp <= n; ++mp) {
-                dc tpm[2] = {{0, 0}, {0, 0}};
-                for (int sgn = 0; sgn < 2; ++sgn) {
-                    const int m = sgn == 0 ? mp : -mp;
-                    if (m - k < -(n - j) || m - k > n - j) continue;
-                    const double coef = ipow_even(abs(m) - abs(m - k) - abs(k)) * Acoef(n - j, m - k) * Acoef(j, k) * rn /
-                                        (((n + j) % 2 ? -1.0 : 1.0) * Acoef(n, m)) * scale;
-                    tpm[sgn] = cscale(Yget(tab, n - j, m - k), coef);
-                }
-                sink.put(n, mp, tpm[0], tpm[1]);
-            }
+        const double rn = pow(rho, n - j);
+        for (int sgn = 0; sgn < 2; ++sgn) {
+            const int m = sgn == 0 ? mp : -mp;
+            if (m - k < -(n - j) || m - k > n - j) continue;
+            const double coef = ipow_even(abs(m) - abs(m - k) - abs(k)) * Acoef(n - j, m - k) * Acoef(j, k) * rn /
+                                (((n + j) % 2 ? -1.0 : 1.0) * Acoef(n, m)) * scale;
+            tpm[sgn] = cscale(Yget(tab, n - j, m - k), coef);
         }
     } else {
         // M2L, offset o (fmm.hpp:287-302); rows of the 27 near offsets stay zero
         const int o = blk - 16;
         const int ox = o % 7 - 3, oy = (o / 7) % 7 - 3, oz = o / 49 - 3;
-        const bool near = ox >= -1 && ox <= 1 && oy >= -1 && oy <= 1 && oz >= -1 && oz <= 1;
-        if (!near) {
-            for (int n = 0; n <= p; ++n) {
-                for (int mp = 0; mp <= n; ++mp) {
-                    dc tpm[2] = {{0, 0}, {0, 0}};
-                    for (int sgn = 0; sgn < 2; ++sgn) {
-                        if (mp == 0 && sgn == 1) continue;
-                        const int m = sgn == 0 ? mp : -mp;
-                        const double coef = ipow_even(abs(k - m) - abs(k) - abs(m)) * Acoef(n, m) * Acoef(j, k) /
-                                            ((n % 2 ? -1.0 : 1.0) * Acoef(j + n, m - k) * pow(rho, j + n + 1));
-                        tpm[sgn] = cscale(Yget(tab, j + n, m - k), coef);
-                    }
-                    sink.put(n, mp, tpm[0], tpm[1]);
-                }
-            }
+        if (ox >= -1 && ox <= 1 && oy >= -1 && oy <= 1 && oz >= -1 && oz <= 1) return;
+        const double rj = pow(rho, j + n + 1);
+        for (int sgn = 0; sgn < 2; ++sgn) {
+            if (mp == 0 && sgn == 1) continue;
+            const int m = sgn == 0 ? mp : -mp;
+            const double coef = ipow_even(abs(k - m) - abs(k) - abs(m)) * Acoef(n, m) * Acoef(j, k) / ((n % 2 ? -1.0 : 1.0) * Acoef(j + n, m - k) * rj);
+            tpm[sgn] = cscale(Yget(tab, j + n, m - k), coef);
         }
     }
-    R *dst = op + (static_cast<size_t>(blk) * K + r) * K;
-    for (int c = 0; c < K; ++c) dst[c] = static_cast<R>(row[c]);
+    // the pair's real-packed columns (each written by this thread only)
+    if (mp == 0) {
+        dst[kidx_re(n, 0) * cs] = static_cast<R>(part == 0 ? tpm[0].re : tpm[0].im);
+    } else {
+        const dc sum = cadd(tpm[0], tpm[1]), dif = csub(tpm[0], tpm[1]);
+        dst[kidx_re(n, mp) * cs] = static_cast<R>(part == 0 ? sum.re : sum.im);
+        dst[kidx_im(n, mp) * cs] = static_cast<R>(part == 0 ? -dif.im : dif.re);
+    }
 }
 
 // ---------------------------------------------------------------------------
 // P2M: one wavefront per leaf; lane i evaluates R_n^m at its particle, the
 // wavefront then sums the 64 rows column by column.
 // ---------------------------------------------------------------------------
+// One wavefront per leaf, 4 per block.  Lane = g (p+1) + m: particle group g
+// (G = 64 / (p+1) groups, particles g, g+G, ...) and order m.  A lane forms
+// R_m^m by the diagonal recurrence and R_n^m, n = m..p, by the upward one,
+// and accumulates m_i conj(R_n^m) in registers; the groups are then summed in
+// fixed order through LDS.
 template <class R>
-__global__ __launch_bounds__(64) void k_p2m(const int32_t *__restrict__ leaves, const int8_t *__restrict__ level, const uint32_t *__restrict__ start,
-                                            const uint32_t *__restrict__ count, const double *__restrict__ bmin, const double *__restrict__ bmax,
-                                            const double *__restrict__ xs64, double wx0, int p, int K, int ldk, R *__restrict__ M) {
-    constexpr int ROWS = 32;
-    __shared__ R rows[ROWS * MAXK];
-    const int32_t c = leaves[blockIdx.x];
+__global__ __launch_bounds__(256) void k_p2m(const int32_t *__restrict__ leaves, int nleaves, const int8_t *__restrict__ level,
+                                             const uint32_t *__restrict__ start, const uint32_t *__restrict__ count, const double *__restrict__ bmin,
+                                             const double *__restrict__ bmax, const double *__restrict__ xs64, double wx0, int p, int K, int ldk,
+                                             R *__restrict__ M) {
+    constexpr int NP = MAXP + 1;
+    __shared__ R red[4][64][2 * NP];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int li = blockIdx.x * 4 + w;
+    if (li >= nleaves) return;  // whole wavefront: no barrier below spans wavefronts
+    const int32_t c = leaves[li];
     if (level[c] < 2) return;  // solver_ref.hpp:187-188
+    const int P1 = p + 1, G = 64 / P1;
+    const int m = lane % P1, g = lane / P1;
+    const bool act = g < G;
     const uint32_t s = start[c], cnt = count[c];
     const double iw = 1.0 / ldexp(wx0, -level[c]);
     const double cx = (bmin[3 * c] + bmax[3 * c]) / 2, cy = (bmin[3 * c + 1] + bmax[3 * c + 1]) / 2, cz = (bmin[3 * c + 2] + bmax[3 * c + 2]) / 2;
-    R acc[3] = {0, 0, 0};
-    const int lane = threadIdx.x;
-    for (uint32_t base = 0; base < cnt; base += ROWS) {
-        if (lane < ROWS) {
-            R *row = rows + lane * K;
-            if (base + lane < cnt) {
-                const uint32_t k = s + base + lane;
-                const R x = static_cast<R>((xs64[4 * k] - cx) * iw), y = static_cast<R>((xs64[4 * k + 1] - cy) * iw),
-                        z = static_cast<R>((xs64[4 * k + 2] - cz) * iw);
-                const R mi = static_cast<R>(xs64[4 * k + 3]);
-                const R r2 = x * x + y * y + z * z;
-                R dre = 1, dim = 0;  // R_m^m
-                for (int m = 0; m <= p; ++m) {
-                    if (m > 0) {
-                        const R d = static_cast<R>(cDiag[m]);
-                        const R nre = d * (x * dre - y * dim), nim = d * (x * dim + y * dre);
-                        dre = nre;
-                        dim = nim;
-                    }
-                    R are = 0, aim = 0, bre = dre, bim = dim;  // R_{n-2}^m, R_{n-1}^m
-                    for (int n = m; n <= p; ++n) {
-                        if (n > m) {
-                            const R c1 = static_cast<R>(cR1[n * (MAXP + 1) + m]), c2 = static_cast<R>(cR2[n * (MAXP + 1) + m]);
-                            const R nre = c1 * z * bre - c2 * r2 * are, nim = c1 * z * bim - c2 * r2 * aim;
-                            are = bre;
-                            aim = bim;
-                            bre = nre;
-                            bim = nim;
-                        }
-                        // M_n^m += m_i conj(R_n^m)
-                        row[kidx_re(n, m)] = mi * bre;
-                        if (m > 0) row[kidx_im(n, m)] = -mi * bim;
-                    }
+    R c1[NP], c2[NP], are[NP], aim[NP];
+#pragma unroll
+    for (int n = 0; n < NP; ++n) {
+        c1[n] = static_cast<R>(cR1[n * (MAXP + 1) + (act ? m : 0)]);
+        c2[n] = static_cast<R>(cR2[n * (MAXP + 1) + (act ? m : 0)]);
+        are[n] = 0;
+        aim[n] = 0;
+    }
+    if (act) {
+        for (uint32_t i = g; i < cnt; i += G) {
+            const uint32_t k = s + i;
+            const R x = static_cast<R>((xs64[4 * k] - cx) * iw), y = static_cast<R>((xs64[4 * k + 1] - cy) * iw),
+                    z = static_cast<R>((xs64[4 * k + 2] - cz) * iw);
+            const R mi = static_cast<R>(xs64[4 * k + 3]);
+            const R r2 = x * x + y * y + z * z;
+            R dre = 1, dim = 0;  // R_m^m
+            for (int q = 1; q <= m; ++q) {
+                const R d = static_cast<R>(cDiag[q]);
+                const R nre = d * (x * dre - y * dim), nim = d * (x * dim + y * dre);
+                dre = nre;
+                dim = nim;
+            }
+            R bre = 0, bim = 0, cre = dre, cim = dim;  // R_{n-2}^m, R_{n-1}^m
+#pragma unroll
+            for (int n = 0; n < NP; ++n) {
+                if (n < m || n > p) continue;
+                if (n > m) {
+                    const R nre = c1[n] * z * cre - c2[n] * r2 * bre, nim = c1[n] * z * cim - c2[n] * r2 * bim;
+                    bre = cre;
+                    bim = cim;
+                    cre = nre;
+                    cim = nim;
                 }
-            } else {
-                for (int q = 0; q < K; ++q) row[q] = 0;
+                are[n] += mi * cre;
+                aim[n] -= mi * cim;  // conj
             }
         }
-        __syncthreads();
-        for (int q = lane, slot = 0; q < K; q += 64, ++slot) {
-            R sum = 0;
-            for (int i = 0; i < ROWS; ++i) sum += rows[i * K + q];
-            acc[slot] += sum;
-        }
-        __syncthreads();
     }
-    for (int q = lane, slot = 0; q < K; q += 64, ++slot) M[static_cast<size_t>(c) * ldk + q] = acc[slot];
+#pragma unroll
+    for (int n = 0; n < NP; ++n) {
+        red[w][lane][2 * n] = are[n];
+        red[w][lane][2 * n + 1] = aim[n];
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane < P1) {  // lane m sums the groups of order m
+        R* out = M + static_cast<size_t>(c) * ldk;
+        for (int n = m; n <= p; ++n) {
+            R sre = 0, sim = 0;
+            for (int gg = 0; gg < G; ++gg) {
+                sre += red[w][gg * P1 + m][2 * n];
+                sim += red[w][gg * P1 + m][2 * n + 1];
+            }
+            out[kidx_re(n, m)] = sre;
+            if (m > 0) out[kidx_im(n, m)] = sim;
+        }
+    }
     for (int q = K + lane; q < ldk; q += 64) M[static_cast<size_t>(c) * ldk + q] = R(0);
 }
 
 // M2M for the internal cells of one level (solver_ref.hpp:196-197).
-template <class R>
-__global__ __launch_bounds__(256) void k_m2m(const int32_t *__restrict__ ids, const int32_t *__restrict__ child, const uint32_t *__restrict__ count,
-                                             const R *__restrict__ op, int K, int ldk, R *__restrict__ M) {
-    __shared__ R cm[8 * MAXK];
-    const int32_t P = ids[blockIdx.x];
-    const int32_t base = child[P];
-    for (int i = threadIdx.x; i < 8 * K; i += 256) {
-        const int q = i / K, col = i % K;
-        cm[i] = count[base + q] ? M[static_cast<size_t>(base + q) * ldk + col] : R(0);
+// M2M / L2L of one tree level, PB parents per block: every operator word read
+// from L2 serves PB parents, and PB is chosen per level (pick_pb) so that the
+// launch still has enough blocks to fill the chip.  Operator blocks are
+// column-major, so the lanes (output rows) read consecutive words; the
+// staged expansions are [col][parent] so a lane reads its PB inputs as vectors.
+template <class R, int PB>
+__global__ __launch_bounds__(256) void k_m2m(const int32_t *__restrict__ ids, int nids, const int32_t *__restrict__ child,
+                                             const uint32_t *__restrict__ count, const R *__restrict__ op, int K, int ldk, R *__restrict__ M) {
+    __shared__ R cm[8 * MAXK][PB];  // [child q][col][parent]
+    const int p0 = blockIdx.x * PB;
+    const int np = min(PB, nids - p0);
+    for (int i = threadIdx.x; i < PB * 8 * K; i += 256) {
+        const int u = i % PB, qc = i / PB;  // qc = q K + col
+        const int q = qc / K, col = qc % K;
+        R v = 0;
+        if (u < np) {
+            const int32_t cb = child[ids[p0 + u]] + q;
+            if (count[cb]) v = M[static_cast<size_t>(cb) * ldk + col];
+        }
+        cm[qc][u] = v;
     }
     __syncthreads();
     for (int r = threadIdx.x; r < K; r += 256) {
         int j = 0;
         while ((j + 1) * (j + 1) <= r) ++j;
         const int cend = (j + 1) * (j + 1);  // output degree j uses child degrees <= j
-        R sum = 0;
+        R acc[PB];
+#pragma unroll
+        for (int u = 0; u < PB; ++u) acc[u] = 0;
         for (int q = 0; q < 8; ++q) {
-            if (!count[base + q]) continue;
-            const R *orow = op + (static_cast<size_t>(q) * K + r) * K;
-            const R *mv = cm + q * K;
-            for (int col = 0; col < cend; ++col) sum += orow[col] * mv[col];
+            const R *ocol = op + static_cast<size_t>(q) * K * K + r;
+            for (int col = 0; col < cend; ++col) {
+                const R a = ocol[static_cast<size_t>(col) * K];
+#pragma unroll
+                for (int u = 0; u < PB; ++u) acc[u] += a * cm[q * K + col][u];  // empty children are zero
+            }
         }
-        M[static_cast<size_t>(P) * ldk + r] = sum;
+#pragma unroll
+        for (int u = 0; u < PB; ++u)
+            if (u < np) M[static_cast<size_t>(ids[p0 + u]) * ldk + r] = acc[u];
     }
-    for (int q = K + threadIdx.x; q < ldk; q += 256) M[static_cast<size_t>(P) * ldk + q] = R(0);
+    for (int i = K + threadIdx.x; i < ldk; i += 256)
+        for (int u = 0; u < np; ++u) M[static_cast<size_t>(ids[p0 + u]) * ldk + i] = R(0);
 }
 
-// L2L from the internal cells of one level to their children (solver_ref.hpp:216-217).
-template <class R>
-__global__ __launch_bounds__(256) void k_l2l(const int32_t *__restrict__ ids, const int32_t *__restrict__ child, const uint32_t *__restrict__ count,
-                                             const R *__restrict__ op, int K, int ldk, R *__restrict__ L) {
-    __shared__ R pl[MAXK];
-    const int32_t P = ids[blockIdx.x];
-    const int32_t base = child[P];
-    for (int i = threadIdx.x; i < K; i += 256) pl[i] = L[static_cast<size_t>(P) * ldk + i];
+template <class R, int PB>
+__global__ __launch_bounds__(256) void k_l2l(const int32_t *__restrict__ ids, int nids, const int32_t *__restrict__ child,
+                                             const uint32_t *__restrict__ count, const R *__restrict__ op, int K, int ldk, R *__restrict__ L) {
+    __shared__ R pl[MAXK][PB];  // [col][parent]
+    const int p0 = blockIdx.x * PB;
+    const int np = min(PB, nids - p0);
+    for (int i = threadIdx.x; i < PB * K; i += 256) {
+        const int u = i % PB, col = i / PB;
+        pl[col][u] = u < np ? L[static_cast<size_t>(ids[p0 + u]) * ldk + col] : R(0);
+    }
     __syncthreads();
     for (int i = threadIdx.x; i < 8 * K; i += 256) {
         const int q = i / K, r = i % K;
-        if (!count[base + q]) continue;
-        const R *orow = op + (static_cast<size_t>(8 + q) * K + r) * K;
+        const R *ocol = op + static_cast<size_t>(8 + q) * K * K + r;
         int j = 0;
         while ((j + 1) * (j + 1) <= r) ++j;
-        R sum = 0;
-        for (int col = j * j; col < K; ++col) sum += orow[col] * pl[col];  // parent degrees >= j
-        L[static_cast<size_t>(base + q) * ldk + r] += sum;
+        R acc[PB];
+#pragma unroll
+        for (int u = 0; u < PB; ++u) acc[u] = 0;
+        for (int col = j * j; col < K; ++col) {  // parent degrees >= j
+            const R a = ocol[static_cast<size_t>(col) * K];
+#pragma unroll
+            for (int u = 0; u < PB; ++u) acc[u] += a * pl[col][u];
+        }
+#pragma unroll
+        for (int u = 0; u < PB; ++u) {
+            if (u >= np) continue;
+            const int32_t cc = child[ids[p0 + u]] + q;
+            if (count[cc]) L[static_cast<size_t>(cc) * ldk + r] += acc[u];
+        }
     }
 }
+
+
 
 // ---------------------------------------------------------------------------
 // Plain M2L: targets are the children of octant qt of a block of internal
@@ -413,15 +458,26 @@ __global__ __launch_bounds__(256) void k_m2l_plain(const int32_t *__restrict__ p
 // L2P: one thread per particle of a leaf; the R_n^m recurrence feeds the
 // potential, d/dz and (d/dx - i d/dy) directly, so no table is stored.
 // ---------------------------------------------------------------------------
-template <class R>
+// Recurrence and ladder coefficients as functions of (n, m): with p a template
+// parameter and the loops unrolled they fold to immediates (the same values
+// as the __constant__ tables, which serve the kernels with a runtime order).
+__device__ __forceinline__ double k_diag(int m) { return -sqrt((2.0 * m - 1) / (2.0 * m)); }
+__device__ __forceinline__ double k_r1(int n, int m) { return (2.0 * n - 1) / sqrt(static_cast<double>(n - m) * (n + m)); }
+__device__ __forceinline__ double k_r2(int n, int m) { return sqrt(static_cast<double>(n + m - 1) * (n - m - 1) / (static_cast<double>(n - m) * (n + m))); }
+__device__ __forceinline__ double k_lz(int l, int m) { return m <= l ? sqrt(static_cast<double>(l + m) * (l - m)) : 0.0; }
+__device__ __forceinline__ double k_la(int l, int m) { return m <= l ? sqrt(static_cast<double>(l + m) * (l + m - 1)) : 0.0; }
+__device__ __forceinline__ double k_lb(int l, int m) { return m + 1 <= l ? sqrt(static_cast<double>(l - m) * (l - m - 1)) : 0.0; }
+
+template <class R, int P>
 __global__ __launch_bounds__(64) void k_l2p(const int32_t *__restrict__ leaves, const int8_t *__restrict__ level, const uint32_t *__restrict__ start,
                                             const uint32_t *__restrict__ count, const double *__restrict__ bmin, const double *__restrict__ bmax,
-                                            const double *__restrict__ xs64, double wx0, int p, int K, int ldk, const R *__restrict__ Lx,
+                                            const double *__restrict__ xs64, double wx0, int ldk, const R *__restrict__ Lx,
                                             double gscale, R *__restrict__ acc4, const uint8_t *__restrict__ skip) {
     __shared__ R sl[MAXK];
     if (skip && skip[blockIdx.x]) return;  // the leaf takes the closure's fallback instead
     const int32_t c = leaves[blockIdx.x];
     if (level[c] < 2) return;  // solver_ref.hpp:220-221
+    constexpr int K = (P + 1) * (P + 1);
     for (int i = threadIdx.x; i < K; i += 64) sl[i] = Lx[static_cast<size_t>(c) * ldk + i];
     __syncthreads();
     const uint32_t s = start[c], cnt = count[c];
@@ -434,17 +490,19 @@ __global__ __launch_bounds__(64) void k_l2p(const int32_t *__restrict__ leaves, 
         const R r2 = x * x + y * y + z * z;
         R phi = 0, gz = 0, gre = 0, gim = 0;  // potential, d/dz, (d/dx - i d/dy)
         R dre = 1, dim = 0;
-        for (int m = 0; m <= p; ++m) {
+#pragma unroll
+        for (int m = 0; m <= P; ++m) {
             if (m > 0) {
-                const R d = static_cast<R>(cDiag[m]);
+                const R d = static_cast<R>(k_diag(m));
                 const R nre = d * (x * dre - y * dim), nim = d * (x * dim + y * dre);
                 dre = nre;
                 dim = nim;
             }
             R are = 0, aim = 0, cre = dre, cim = dim;  // R_{n-2}^m, R_n^m
-            for (int n = m; n <= p; ++n) {
+#pragma unroll
+            for (int n = m; n <= P; ++n) {
                 if (n > m) {
-                    const R c1 = static_cast<R>(cR1[n * (MAXP + 1) + m]), c2 = static_cast<R>(cR2[n * (MAXP + 1) + m]);
+                    const R c1 = static_cast<R>(k_r1(n, m)), c2 = static_cast<R>(k_r2(n, m));
                     const R nre = c1 * z * cre - c2 * r2 * are, nim = c1 * z * cim - c2 * r2 * aim;
                     are = cre;
                     aim = cim;
@@ -456,21 +514,21 @@ __global__ __launch_bounds__(64) void k_l2p(const int32_t *__restrict__ leaves, 
                     phi += sl[kidx_re(n, 0)] * cre;
                 else
                     phi += 2 * (sl[kidx_re(n, m)] * cre - sl[kidx_im(n, m)] * cim);
-                if (n + 1 <= p) {
+                if (n + 1 <= P) {
                     const int l = n + 1;  // R_n^m enters the derivatives of degree-l terms
-                    const R lz = static_cast<R>(cLz[l * (MAXP + 2) + m]);
+                    const R lz = static_cast<R>(k_lz(l, m));
                     if (m == 0)
                         gz += sl[kidx_re(l, 0)] * lz * cre;
                     else
                         gz += 2 * lz * (sl[kidx_re(l, m)] * cre - sl[kidx_im(l, m)] * cim);
                     {  // -sqrt((l+m+1)(l+m)) L_l^{m+1} R_n^m
-                        const R la = static_cast<R>(cLa[l * (MAXP + 2) + m + 1]);
+                        const R la = static_cast<R>(k_la(l, m + 1));
                         const R Lre = sl[kidx_re(l, m + 1)], Lim = sl[kidx_im(l, m + 1)];
                         gre -= la * (Lre * cre - Lim * cim);
                         gim -= la * (Lre * cim + Lim * cre);
                     }
                     if (m >= 1) {  // sqrt((l-m+1)(l-m)) conj(L_l^{m-1} R_n^m), with Re(L_l^0) for m = 1
-                        const R lb = static_cast<R>(cLb[l * (MAXP + 2) + m - 1]);
+                        const R lb = static_cast<R>(k_lb(l, m - 1));
                         const R Lre = sl[kidx_re(l, m - 1)], Lim = (m - 1) == 0 ? R(0) : sl[kidx_im(l, m - 1)];
                         gre += lb * (Lre * cre - Lim * cim);
                         gim -= lb * (Lre * cim + Lim * cre);
@@ -511,17 +569,50 @@ struct Bufs<float> {
     }
 };
 
+inline int pick_pb(int64_t nparents, int maxpb) {
+    int pb = 1;
+    while (pb < maxpb && nparents >= static_cast<int64_t>(pb) * 2 * 512) pb *= 2;
+    return pb;
+}
+
+template <class R>
+void m2m_level(Ctx &c, const int32_t *ids, int64_t n) {
+    const TreeDev &t = c.t;
+    const int pb = pick_pb(n, 8);  // 8 children x PB staged expansions in LDS
+#define VG_M2M(PB_) \
+    k_m2m<R, PB_><<<ceil_div(n, PB_), 256, 0, c.stream>>>(ids, static_cast<int>(n), t.child, t.count, Bufs<R>::op(c), c.K, c.ldk, Bufs<R>::M(c))
+    if (pb == 1) VG_M2M(1);
+    else if (pb == 2) VG_M2M(2);
+    else if (pb == 4) VG_M2M(4);
+    else VG_M2M(8);
+#undef VG_M2M
+}
+
+template <class R>
+void l2l_level(Ctx &c, const int32_t *ids, int64_t n) {
+    const TreeDev &t = c.t;
+    const int pb = pick_pb(n, 16);
+#define VG_L2L(PB_) \
+    k_l2l<R, PB_><<<ceil_div(n, PB_), 256, 0, c.stream>>>(ids, static_cast<int>(n), t.child, t.count, Bufs<R>::op(c), c.K, c.ldk, Bufs<R>::L(c))
+    if (pb == 1) VG_L2L(1);
+    else if (pb == 2) VG_L2L(2);
+    else if (pb == 4) VG_L2L(4);
+    else if (pb == 8) VG_L2L(8);
+    else VG_L2L(16);
+#undef VG_L2L
+}
+
 template <class R>
 void upward_t(Ctx &c) {
     TreeDev &t = c.t;
     Bufs<R>::ensure(c, t.ncells);
     const double wx0 = t.box[3] - t.box[0];
     if (t.nleaves)
-        k_p2m<R><<<static_cast<unsigned>(t.nleaves), 64, 0, c.stream>>>(t.leaf_ids, t.level, t.start, t.count, t.bmin, t.bmax, c.xs64, wx0, c.p,
-                                                                       c.K, c.ldk, Bufs<R>::M(c));
+        k_p2m<R><<<ceil_div(t.nleaves, 4), 256, 0, c.stream>>>(t.leaf_ids, static_cast<int>(t.nleaves), t.level, t.start, t.count, t.bmin, t.bmax,
+                                                               c.xs64, wx0, c.p, c.K, c.ldk, Bufs<R>::M(c));
     for (int d = t.depth; d >= 2; --d) {
         const int64_t a = t.int_off[d], b = t.int_off[d + 1];
-        if (b > a) k_m2m<R><<<static_cast<unsigned>(b - a), 256, 0, c.stream>>>(t.int_ids.p + a, t.child, t.count, Bufs<R>::op(c), c.K, c.ldk, Bufs<R>::M(c));
+        if (b > a) m2m_level<R>(c, t.int_ids.p + a, b - a);
     }
     VG_LAUNCH_CHECK();
 }
@@ -551,9 +642,27 @@ void l2p_t(Ctx &c, const int32_t *leaves, int64_t n, const uint8_t *skip) {
     const TreeDev &t = c.t;
     const double wx0 = t.box[3] - t.box[0];
     const double gscale = (c.flags & 1u) ? 1.0 : c.G;
-    if (n)
-        k_l2p<R><<<static_cast<unsigned>(n), 64, 0, c.stream>>>(leaves, t.level, t.start, t.count, t.bmin, t.bmax, c.xs64, wx0, c.p, c.K, c.ldk,
-                                                                Bufs<R>::L(c), gscale, Bufs<R>::acc(c), skip);
+    if (!n) return;
+    const unsigned grid = static_cast<unsigned>(n);
+#define VG_L2P(P_)                                                                                                                            \
+    k_l2p<R, P_><<<grid, 64, 0, c.stream>>>(leaves, t.level, t.start, t.count, t.bmin, t.bmax, c.xs64, wx0, c.ldk, Bufs<R>::L(c), gscale, \
+                                             Bufs<R>::acc(c), skip)
+    switch (c.p) {
+        case 1: VG_L2P(1); break;
+        case 2: VG_L2P(2); break;
+        case 3: VG_L2P(3); break;
+        case 4: VG_L2P(4); break;
+        case 5: VG_L2P(5); break;
+        case 6: VG_L2P(6); break;
+        case 7: VG_L2P(7); break;
+        case 8: VG_L2P(8); break;
+        case 9: VG_L2P(9); break;
+        case 10: VG_L2P(10); break;
+        case 11: VG_L2P(11); break;
+        case 12: VG_L2P(12); break;
+        default: throw Error(1, "L2P supports 1 <= p <= 12");
+    }
+#undef VG_L2P
     VG_LAUNCH_CHECK();
 }
 
@@ -562,7 +671,7 @@ void downward_t(Ctx &c) {
     TreeDev &t = c.t;
     for (int d = 2; d < t.depth; ++d) {  // parents at level d >= 2 feed children at level >= 3
         const int64_t a = t.int_off[d], b = t.int_off[d + 1];
-        if (b > a) k_l2l<R><<<static_cast<unsigned>(b - a), 256, 0, c.stream>>>(t.int_ids.p + a, t.child, t.count, Bufs<R>::op(c), c.K, c.ldk, Bufs<R>::L(c));
+        if (b > a) l2l_level<R>(c, t.int_ids.p + a, b - a);
     }
     bool fallback = false;
     if constexpr (sizeof(R) == 4) {
@@ -627,7 +736,8 @@ void operators_build(Ctx &c) {
     const double wx = t.box[3] - t.box[0], wy = t.box[4] - t.box[1], wz = t.box[5] - t.box[2];
     const double ry = wx > 0 ? wy / wx : 1.0, rz = wx > 0 ? wz / wx : 1.0;
     // offset vectors in units of the x width of the level: 8 M2M (child - parent), 8 L2L (parent - child), 343 M2L (source - target)
-    std::vector<double> vec(3 * (16 + 343));
+    std::vector<double> &vec = c.opvec_host;  // kept alive with the context: the upload below is asynchronous
+    vec.assign(3 * (16 + 343), 0.0);
     for (int q = 0; q < 8; ++q) {
         const double vx = (q & 1) ? -0.25 : 0.25, vy = ((q >> 1) & 1) ? -0.25 * ry : 0.25 * ry, vz = ((q >> 2) & 1) ? -0.25 * rz : 0.25 * rz;
         vec[3 * q] = vx;
@@ -643,26 +753,24 @@ void operators_build(Ctx &c) {
         vec[3 * (16 + o) + 2] = (o / 49 - 3) * rz;
     }
     const int nvec = 16 + 343;
-    DevBuf<double> dvec;
-    dvec.ensure(vec.size());
-    VG_HIP(hipMemcpyAsync(dvec.p, vec.data(), vec.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    c.opvec.ensure(vec.size());
+    VG_HIP(hipMemcpyAsync(c.opvec.p, vec.data(), vec.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
     c.harm.ensure(static_cast<size_t>(nvec) * HSTRIDE);
-    k_harmonics<<<ceil_div(nvec, 64), 64, 0, c.stream>>>(dvec, nvec, 2 * c.p, c.harm);
-    dim3 grid(nvec, ceil_div(c.K, 64));
+    k_harmonics<<<nvec, 64, 0, c.stream>>>(c.opvec, nvec, 2 * c.p, c.harm);
+    dim3 grid(nvec, c.K);
     if (c.prec == 0) {
         c.op64.ensure(static_cast<size_t>(nvec) * c.K * c.K);
-        k_ops<double><<<grid, 64, 0, c.stream>>>(c.harm, c.p, c.K, c.op64);
+        k_ops<double><<<grid, 128, 0, c.stream>>>(c.harm, c.p, c.K, c.op64);
     } else if (c.prec == 2) {
         c.op64.ensure(static_cast<size_t>(nvec) * c.K * c.K);
-        k_ops<double><<<grid, 64, 0, c.stream>>>(c.harm, c.p, c.K, c.op64);
+        k_ops<double><<<grid, 128, 0, c.stream>>>(c.harm, c.p, c.K, c.op64);
         c.op32.ensure(static_cast<size_t>(nvec) * c.K * c.K);
         m2l_split_operators(c, nvec);
     } else {
         c.op32.ensure(static_cast<size_t>(nvec) * c.K * c.K);
-        k_ops<float><<<grid, 64, 0, c.stream>>>(c.harm, c.p, c.K, c.op32);
+        k_ops<float><<<grid, 128, 0, c.stream>>>(c.harm, c.p, c.K, c.op32);
     }
     VG_LAUNCH_CHECK();
-    VG_HIP(hipStreamSynchronize(c.stream));  // dvec is released on return
 }
 
 void fmm_upward(Ctx &c) {
@@ -687,7 +795,7 @@ void fmm_m2l_plain_list(Ctx &c, const int32_t *parents, int np) {
 }
 
 void l2l_list(Ctx &c, const int32_t *parents, int64_t np) {
-    if (np > 0) k_l2l<float><<<static_cast<unsigned>(np), 256, 0, c.stream>>>(parents, c.t.child, c.t.count, c.op32, c.K, c.ldk, c.L32);
+    if (np > 0) l2l_level<float>(c, parents, np);
     VG_LAUNCH_CHECK();
 }
 

@@ -129,6 +129,8 @@ struct Ctx {
     DevBuf<float> op32;
     DevBuf<uint16_t> opsplit;   // M2L operators as BF16 hi/lo MFMA fragments
     DevBuf<double> consts;      // A_n^m and recurrence constants (device copy)
+    DevBuf<double> opvec;       // operator offset vectors (device copy)
+    std::vector<double> opvec_host;
 
     // ref-numbering scratch
     DevBuf<uint32_t> slist;
@@ -136,7 +138,8 @@ struct Ctx {
     DevBuf<uint32_t> rval[2];
 
     // P2P work items
-    DevBuf<uint32_t> p2p_scal, p2p_nchunk, p2p_item_off;
+    DevBuf<uint32_t> p2p_scal, p2p_nchunk, p2p_item_off, p2p_local, p2p_pre, p2p_btot;
+    DevBuf<int32_t> p2p_split;  // leaves whose sources span several chunks
     DevBuf<int64_t> p2p_items;  // int2 (leaf index, chunk)
     DevBuf<float> p2p_part32;
     DevBuf<double> p2p_part64;

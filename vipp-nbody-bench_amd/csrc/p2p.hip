@@ -220,76 +220,118 @@ __global__ __launch_bounds__(P2P_THREADS) void k_p2p(const int2 *__restrict__ it
 
 // Per leaf: number of sources in its NI chain, split into chunks of at most
 // P2P_CHUNK sources (bounded work per block), and its class by targets per lane.
+// Sources of each cell's own NI entries (the leaf's full list is its
+// ancestors' entries and its own, solver_ref.hpp:126).
+__global__ __launch_bounds__(256) void k_ni_local(int64_t ncells, const uint32_t *__restrict__ count, const int32_t *__restrict__ ni,
+                                                  const int32_t *__restrict__ ni_cnt, uint32_t *__restrict__ local) {
+    const int64_t c = blockIdx.x * 256ll + threadIdx.x;
+    if (c >= ncells) return;
+    const int ne = ni_cnt[c];
+    uint32_t s = 0;
+    for (int e = 0; e < ne; ++e) s += count[ni[27 * c + e]];
+    local[c] = s;
+}
+
+constexpr int NSC = 6;  // scanned per leaf: chunks, chunks of class 0..3, split flag
+__device__ inline int leaf_class(uint32_t nt) { return nt <= 64 ? 0 : (nt <= 128 ? 1 : (nt <= 192 ? 2 : 3)); }
+
+// Per leaf: source count -> chunks and class; block-local exclusive scans of
+// the NSC components (one leaf per thread, 256 per block) and block totals.
 __global__ __launch_bounds__(256) void k_p2p_plan(const int32_t *__restrict__ leaves, int nleaves, const int8_t *__restrict__ level,
-                                                  const uint32_t *__restrict__ count, const int32_t *__restrict__ anc, const int32_t *__restrict__ ni,
-                                                  const int32_t *__restrict__ ni_cnt, uint32_t *__restrict__ nchunk, uint32_t *__restrict__ cls_cnt) {
+                                                  const uint32_t *__restrict__ count, const int32_t *__restrict__ anc, const uint32_t *__restrict__ local,
+                                                  uint32_t *__restrict__ nchunk, uint32_t *__restrict__ pre, uint32_t *__restrict__ btot) {
+    __shared__ uint32_t wtot[4][NSC];
     const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= nleaves) return;
-    const int32_t c = leaves[k];
-    uint32_t src = 0;
-    for (int l = 0; l <= level[c]; ++l) {
-        const int32_t a = anc[static_cast<int64_t>(c) * (MAXD + 1) + l];
-        for (int e = 0; e < ni_cnt[a]; ++e) src += count[ni[27 * static_cast<int64_t>(a) + e]];
+    const unsigned lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t v[NSC] = {0, 0, 0, 0, 0, 0};
+    if (k < nleaves) {
+        const int32_t c = leaves[k];
+        uint32_t src = 0;
+        for (int l = 0; l <= level[c]; ++l) src += local[anc[static_cast<int64_t>(c) * (MAXD + 1) + l]];
+        const uint32_t nch = max(1u, (src + P2P_CHUNK - 1) / P2P_CHUNK);
+        nchunk[k] = nch;
+        v[0] = nch;
+        v[1 + leaf_class(count[c])] = nch;
+        v[5] = nch > 1 ? 1u : 0u;
     }
-    const uint32_t nch = max(1u, (src + P2P_CHUNK - 1) / P2P_CHUNK);
-    nchunk[k] = nch;
-    const uint32_t nt = count[c];
-    const int cls = nt <= 64 ? 0 : (nt <= 128 ? 1 : (nt <= 192 ? 2 : 3));
-    atomicAdd(&cls_cnt[cls], nch);
-}
-
-// Exclusive scan in one block (item offsets of the leaves); total in *sum.
-__global__ __launch_bounds__(1024) void k_scan_excl(const uint32_t *__restrict__ in, int n, uint32_t *__restrict__ out, uint32_t *__restrict__ sum) {
-    __shared__ uint32_t wsum[16];
-    __shared__ uint32_t carry;
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    const unsigned lane = threadIdx.x & 63;
-    for (int off = 0; off < n; off += 1024) {
-        const int i = off + threadIdx.x;
-        const uint32_t v = i < n ? in[i] : 0u;
-        uint32_t s = v;
+    uint32_t inc[NSC];
+#pragma unroll
+    for (int q = 0; q < NSC; ++q) {
+        uint32_t x = v[q];
         for (int d = 1; d < 64; d <<= 1) {
-            const uint32_t o = __shfl_up(s, d, 64);
-            if (lane >= static_cast<unsigned>(d)) s += o;
+            const uint32_t o = __shfl_up(x, d, 64);
+            if (lane >= static_cast<unsigned>(d)) x += o;
         }
-        if (lane == 63) wsum[threadIdx.x >> 6] = s;
-        __syncthreads();
-        uint32_t wb = 0;
-        for (unsigned w = 0; w < (threadIdx.x >> 6); ++w) wb += wsum[w];
-        const uint32_t c0 = carry;
-        if (i < n) out[i] = c0 + wb + s - v;
-        __syncthreads();
-        if (threadIdx.x == 1023) carry = c0 + wb + s;
-        __syncthreads();
+        inc[q] = x;
+        if (lane == 63) wtot[w][q] = x;
     }
-    if (threadIdx.x == 0) *sum = carry;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NSC; ++q) {
+        uint32_t base = 0;
+        for (unsigned u = 0; u < w; ++u) base += wtot[u][q];
+        if (k < nleaves) pre[static_cast<int64_t>(k) * NSC + q] = base + inc[q] - v[q];
+        if (threadIdx.x == 255) btot[blockIdx.x * NSC + q] = base + inc[q];
+    }
 }
 
-// Items (leaf index, chunk) bucketed by class; cls_base holds the class offsets.
+// One block: exclusive scan of the block totals; writes the class counts and
+// bases, the item total and the split-leaf total into scal.
+//   scal[0..3] items of class 0..3, scal[4..7] class bases, scal[8] items,
+//   scal[9] split leaves.
+__global__ __launch_bounds__(64) void k_p2p_plan_blocks(uint32_t *__restrict__ btot, int nblocks, uint32_t *__restrict__ scal) {
+    __shared__ uint32_t tot[NSC];
+    const int q = threadIdx.x;
+    if (q < NSC) {
+        uint32_t run = 0;
+        for (int b = 0; b < nblocks; ++b) {
+            const uint32_t t = btot[b * NSC + q];
+            btot[b * NSC + q] = run;
+            run += t;
+        }
+        tot[q] = run;
+    }
+    __syncthreads();
+    if (q == 0) {
+        scal[8] = tot[0];
+        scal[9] = tot[5];
+        uint32_t b = 0;
+        for (int j = 0; j < 4; ++j) {
+            scal[j] = tot[1 + j];
+            scal[4 + j] = b;
+            b += tot[1 + j];
+        }
+    }
+}
+
+// Item offsets, items (leaf, chunk) bucketed by class in leaf order, and the
+// list of leaves whose sources span several chunks.
 __global__ __launch_bounds__(256) void k_p2p_items(const int32_t *__restrict__ leaves, int nleaves, const uint32_t *__restrict__ count,
-                                                   const uint32_t *__restrict__ nchunk, uint32_t *__restrict__ cls_fill, uint4 cls_base,
-                                                   int2 *__restrict__ items) {
+                                                   const uint32_t *__restrict__ nchunk, const uint32_t *__restrict__ pre,
+                                                   const uint32_t *__restrict__ boff, const uint32_t *__restrict__ scal,
+                                                   uint32_t *__restrict__ item_off, int2 *__restrict__ items, int32_t *__restrict__ split_list) {
     const int k = blockIdx.x * 256 + threadIdx.x;
     if (k >= nleaves) return;
-    const uint32_t nt = count[leaves[k]];
-    const int cls = nt <= 64 ? 0 : (nt <= 128 ? 1 : (nt <= 192 ? 2 : 3));
+    const uint32_t *pk = pre + static_cast<int64_t>(k) * NSC;
+    const uint32_t *bk = boff + blockIdx.x * NSC;
     const uint32_t nch = nchunk[k];
-    const uint32_t base = cls == 0 ? cls_base.x : (cls == 1 ? cls_base.y : (cls == 2 ? cls_base.z : cls_base.w));
-    const uint32_t pos = base + atomicAdd(&cls_fill[cls], nch);
+    item_off[k] = bk[0] + pk[0];
+    const int cls = leaf_class(count[leaves[k]]);
+    const uint32_t pos = scal[4 + cls] + bk[1 + cls] + pk[1 + cls];
     for (uint32_t q = 0; q < nch; ++q) items[pos + q] = make_int2(k, static_cast<int>(q));
+    if (nch > 1) split_list[bk[5] + pk[5]] = k;
 }
 
 // Sum the chunk partials of split leaves in chunk order.
 template <class R, bool SELF_GUARD, bool POT>
-__global__ __launch_bounds__(64) void k_p2p_reduce(const int32_t *__restrict__ leaves, const uint32_t *__restrict__ nchunk, const uint32_t *__restrict__ item_off,
+__global__ __launch_bounds__(64) void k_p2p_reduce(const int32_t *__restrict__ split_list, const int32_t *__restrict__ leaves,
+                                                   const uint32_t *__restrict__ nchunk, const uint32_t *__restrict__ item_off,
                                                    const uint32_t *__restrict__ start, const uint32_t *__restrict__ count,
                                                    const typename Vec4<R>::T *__restrict__ xs, R eps2, R G, const typename Vec4<R>::T *__restrict__ partial,
                                                    uint32_t pstride, typename Vec4<R>::T *__restrict__ acc) {
     using V4 = typename Vec4<R>::T;
-    const int k = blockIdx.x;
+    const int k = split_list[blockIdx.x];
     const uint32_t nch = nchunk[k];
-    if (nch <= 1) return;
     const int32_t c = leaves[k];
     const uint32_t nt = count[c], pt0 = start[c];
     for (uint32_t t = threadIdx.x; t < nt; t += 64) {
@@ -323,29 +365,28 @@ void p2p_t(Ctx &c, bool potential) {
     const R eps2 = static_cast<R>(c.eps * c.eps);
     const bool guard = !(eps2 > 0);
     const int nl = static_cast<int>(t.nleaves);
+    const int nb = static_cast<int>(ceil_div(nl, 256));
     c.p2p_scal.ensure(16);
     c.p2p_nchunk.ensure(nl);
     c.p2p_item_off.ensure(nl);
-    uint32_t *cls_cnt = c.p2p_scal.p, *cls_fill = c.p2p_scal.p + 4, *nitems_d = c.p2p_scal.p + 12;
-    VG_HIP(hipMemsetAsync(c.p2p_scal.p, 0, 16 * sizeof(uint32_t), c.stream));
-    k_p2p_plan<<<ceil_div(nl, 256), 256, 0, c.stream>>>(t.leaf_ids, nl, t.level, t.count, t.anc, t.ni, t.ni_cnt, c.p2p_nchunk, cls_cnt);
-    k_scan_excl<<<1, 1024, 0, c.stream>>>(c.p2p_nchunk, nl, c.p2p_item_off, nitems_d);
+    c.p2p_local.ensure(t.ncells);
+    c.p2p_pre.ensure(static_cast<size_t>(nl) * NSC);
+    c.p2p_btot.ensure(static_cast<size_t>(nb) * NSC);
+    c.p2p_split.ensure(nl);
+    k_ni_local<<<ceil_div(t.ncells, 256), 256, 0, c.stream>>>(t.ncells, t.count, t.ni, t.ni_cnt, c.p2p_local);
+    k_p2p_plan<<<nb, 256, 0, c.stream>>>(t.leaf_ids, nl, t.level, t.count, t.anc, c.p2p_local, c.p2p_nchunk, c.p2p_pre, c.p2p_btot);
+    k_p2p_plan_blocks<<<1, 64, 0, c.stream>>>(c.p2p_btot, nb, c.p2p_scal);
     VG_LAUNCH_CHECK();
     uint32_t hs[16];
     VG_HIP(hipMemcpyAsync(hs, c.p2p_scal.p, sizeof(hs), hipMemcpyDeviceToHost, c.stream));
     VG_HIP(hipStreamSynchronize(c.stream));
-    const uint32_t nitems = hs[12];
-    uint32_t hbase[4], acc_base = 0;
-    for (int q = 0; q < 4; ++q) {
-        hbase[q] = acc_base;
-        acc_base += hs[q];
-    }
+    const uint32_t nitems = hs[8], nsplit = hs[9];
+    const uint32_t *hbase = hs + 4;
     c.p2p_items.ensure(std::max<uint32_t>(1, nitems));
     int2 *items = reinterpret_cast<int2 *>(c.p2p_items.p);
-    k_p2p_items<<<ceil_div(nl, 256), 256, 0, c.stream>>>(t.leaf_ids, nl, t.count, c.p2p_nchunk, cls_fill, make_uint4(hbase[0], hbase[1], hbase[2], hbase[3]),
-                                                         items);
+    k_p2p_items<<<nb, 256, 0, c.stream>>>(t.leaf_ids, nl, t.count, c.p2p_nchunk, c.p2p_pre, c.p2p_btot, c.p2p_scal, c.p2p_item_off, items, c.p2p_split);
     VG_LAUNCH_CHECK();
-    const bool split = nitems > static_cast<uint32_t>(nl);
+    const bool split = nsplit > 0;
     const uint32_t pstride = c.thr;
     V4 *partial = nullptr;
     if (split) {
@@ -391,7 +432,8 @@ void p2p_t(Ctx &c, bool potential) {
     }
     if (split) {
 #define VG_RED(G_, P_) \
-    k_p2p_reduce<R, G_, P_><<<nl, 64, 0, c.stream>>>(t.leaf_ids, c.p2p_nchunk, c.p2p_item_off, t.start, t.count, xs, eps2, static_cast<R>(c.G), partial, pstride, acc)
+    k_p2p_reduce<R, G_, P_><<<nsplit, 64, 0, c.stream>>>(c.p2p_split, t.leaf_ids, c.p2p_nchunk, c.p2p_item_off, t.start, t.count, xs, eps2, \
+                                                         static_cast<R>(c.G), partial, pstride, acc)
         if (guard) {
             if (potential)
                 VG_RED(true, true);

@@ -56,21 +56,29 @@ __global__ __launch_bounds__(256) void k_aabb(const double *__restrict__ x, int6
 }
 
 // final reduction; box = padded root bounds (fmm.hpp:403-405), box[6] = non-finite flag
-__global__ void k_aabb_final(const double *__restrict__ part, int nb, double *__restrict__ box) {
-    if (threadIdx.x != 0) return;
-    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY}, bad = 0;
-    for (int b = 0; b < nb; ++b) {
+__global__ __launch_bounds__(256) void k_aabb_final(const double *__restrict__ part, int nb, double *__restrict__ box) {
+    __shared__ double s[7][256];
+    double v[7] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, 0};
+    for (int b = threadIdx.x; b < nb; b += 256) {
         for (int a = 0; a < 3; ++a) {
-            lo[a] = fmin(lo[a], part[7 * b + a]);
-            hi[a] = fmax(hi[a], part[7 * b + 3 + a]);
+            v[a] = fmin(v[a], part[7 * b + a]);
+            v[3 + a] = fmax(v[3 + a], part[7 * b + 3 + a]);
         }
-        bad = fmax(bad, part[7 * b + 6]);
+        v[6] = fmax(v[6], part[7 * b + 6]);
     }
-    for (int a = 0; a < 3; ++a) {
-        box[a] = lo[a] - PADDING;
-        box[3 + a] = hi[a] + PADDING;
+    for (int a = 0; a < 7; ++a) s[a][threadIdx.x] = v[a];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < static_cast<unsigned>(w))
+            for (int a = 0; a < 7; ++a)
+                s[a][threadIdx.x] = a < 3 ? fmin(s[a][threadIdx.x], s[a][threadIdx.x + w]) : fmax(s[a][threadIdx.x], s[a][threadIdx.x + w]);
+        __syncthreads();
     }
-    box[6] = bad;
+    if (threadIdx.x < 3) {
+        box[threadIdx.x] = s[threadIdx.x][0] - PADDING;          // min and max are exact: the order of the reduction does not matter
+        box[3 + threadIdx.x] = s[3 + threadIdx.x][0] + PADDING;
+    }
+    if (threadIdx.x == 0) box[6] = s[6][0];
 }
 
 // 63-bit path code: level-1 octant in bits 62..60, level-21 octant in bits 2..0.
@@ -386,7 +394,7 @@ void tree_build(Ctx &c) {
     c.box.ensure(8);
     c.scal.ensure(16);
     k_aabb<<<nb, 256, 0, st>>>(c.x, n, c.aabb_part);
-    k_aabb_final<<<1, 64, 0, st>>>(c.aabb_part, nb, c.box);
+    k_aabb_final<<<1, 256, 0, st>>>(c.aabb_part, nb, c.box);
     VG_LAUNCH_CHECK();
     // 2. path codes and stable sort
     for (int b = 0; b < 2; ++b) {
