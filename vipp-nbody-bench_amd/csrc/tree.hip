@@ -262,46 +262,87 @@ __global__ void k_lists_root(const int32_t *__restrict__ child, int32_t *nb, int
     ap_cnt[0] = 0;
 }
 
-__global__ __launch_bounds__(128) void k_lists_level(int64_t c0, int64_t c1, const int32_t *__restrict__ parent, const int32_t *__restrict__ child,
+// Lists of the 8 children of one internal cell per wavefront (4 per block).
+// The siblings share their candidates: the children of the parent's NB
+// cells, taken in the order of solver_ref.hpp:129-147 (NB entry e, child
+// offset q).  Lane l tests candidate 64 r + l of round r against all 8
+// siblings; each sibling's NB / NI / AP entries are appended in candidate
+// order through ballots and prefix popcounts, so the lists come out in the
+// reference's order.
+__global__ __launch_bounds__(256) void k_lists_level(const int32_t *__restrict__ pids, int np, const int32_t *__restrict__ child,
                                                      const int32_t *__restrict__ grid, int32_t *__restrict__ nb, int32_t *__restrict__ ni,
                                                      int32_t *__restrict__ ap, int32_t *__restrict__ nb_cnt, int32_t *__restrict__ ni_cnt,
                                                      int32_t *__restrict__ ap_cnt, int32_t *__restrict__ nbr27, uint32_t *__restrict__ overflow) {
-    const int64_t c = c0 + blockIdx.x * 128ll + threadIdx.x;
-    if (c >= c1) return;
-    const int32_t P = parent[c];
-    const int gx = grid[3 * c], gy = grid[3 * c + 1], gz = grid[3 * c + 2];
-    const bool leaf = child[c] < 0;
-    int nbn = 0, nin = 0, apn = 0;
-    int32_t *nbc = nb + 27 * c, *nic = ni + 27 * c, *apc = ap + 189 * c;
-    int32_t *n27 = nbr27 + 27 * c;
-    for (int q = 0; q < 27; ++q) n27[q] = -1;
-    const int np = nb_cnt[P];
-    for (int e = 0; e < np; ++e) {  // solver_ref.hpp:129-147
-        const int32_t pn = nb[27 * static_cast<int64_t>(P) + e];
-        const int32_t base = child[pn];
-        for (int q = 0; q < 8; ++q) {
-            const int32_t cc = base + q;
-            const int dx = grid[3 * cc] - gx, dy = grid[3 * cc + 1] - gy, dz = grid[3 * cc + 2] - gz;
+    const int wv = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wv >= np) return;  // whole wavefront
+    const unsigned lane = threadIdx.x & 63;
+    const uint64_t below = (1ull << lane) - 1;
+    const int32_t P = pids[wv];
+    const int32_t cb = child[P];
+    const int npn = nb_cnt[P];
+    for (int i = lane; i < 8 * 27; i += 64) nbr27[27 * static_cast<int64_t>(cb) + i] = -1;
+    int gx[8], gy[8], gz[8];
+    bool leaf[8];
+    int nbn[8], nin[8], apn[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        gx[t] = grid[3 * (cb + t)];
+        gy[t] = grid[3 * (cb + t) + 1];
+        gz[t] = grid[3 * (cb + t) + 2];
+        leaf[t] = child[cb + t] < 0;
+        nbn[t] = nin[t] = apn[t] = 0;
+    }
+    const int ncand = 8 * npn;
+    for (int r0 = 0; r0 < ncand; r0 += 64) {
+        const int cand = r0 + static_cast<int>(lane);
+        const bool valid = cand < ncand;
+        int32_t cc = 0;
+        int cx = 0, cy = 0, cz = 0;
+        bool cint = false;
+        if (valid) {
+            cc = child[nb[27 * static_cast<int64_t>(P) + cand / 8]] + cand % 8;
+            cx = grid[3 * cc];
+            cy = grid[3 * cc + 1];
+            cz = grid[3 * cc + 2];
+            cint = child[cc] >= 0;
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int32_t ct = cb + t;
+            const int dx = cx - gx[t], dy = cy - gy[t], dz = cz - gz[t];
             const bool adj = dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1;  // fmm.hpp:475-493
-            if (adj) {
-                if (!leaf && child[cc] >= 0) {
-                    if (nbn < 27) nbc[nbn] = cc;
-                    ++nbn;
-                    n27[(dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)] = cc;
-                } else {
-                    if (nin < 27) nic[nin] = cc;
-                    ++nin;
-                }
-            } else {
-                if (apn < 189) apc[apn] = cc;
-                ++apn;
+            const bool isnb = valid && adj && !leaf[t] && cint;
+            const bool isni = valid && adj && !isnb;
+            const bool isap = valid && !adj;
+            const uint64_t mnb = __ballot(isnb), mni = __ballot(isni), map = __ballot(isap);
+            if (isnb) {
+                const int k = nbn[t] + __popcll(mnb & below);
+                if (k < 27) nb[27 * static_cast<int64_t>(ct) + k] = cc;
+                nbr27[27 * static_cast<int64_t>(ct) + (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)] = cc;
             }
+            if (isni) {
+                const int k = nin[t] + __popcll(mni & below);
+                if (k < 27) ni[27 * static_cast<int64_t>(ct) + k] = cc;
+            }
+            if (isap) {
+                const int k = apn[t] + __popcll(map & below);
+                if (k < 189) ap[189 * static_cast<int64_t>(ct) + k] = cc;
+            }
+            nbn[t] += __popcll(mnb);
+            nin[t] += __popcll(mni);
+            apn[t] += __popcll(map);
         }
     }
-    if (nbn > 27 || nin > 27 || apn > 189) atomicOr(overflow, 1u);
-    nb_cnt[c] = min(nbn, 27);
-    ni_cnt[c] = min(nin, 27);
-    ap_cnt[c] = min(apn, 189);
+    if (lane < 8) {
+        int a = 0, b = 0, d = 0;
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+            if (lane == static_cast<unsigned>(t)) a = nbn[t], b = nin[t], d = apn[t];
+        if (a > 27 || b > 27 || d > 189) atomicOr(overflow, 1u);
+        nb_cnt[cb + lane] = min(a, 27);
+        ni_cnt[cb + lane] = min(b, 27);
+        ap_cnt[cb + lane] = min(d, 189);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -501,10 +542,11 @@ void lists_build(Ctx &c) {
     t.nbr27.ensure(27 * nc);
     VG_HIP(hipMemsetAsync(c.scal.p + 4, 0, sizeof(uint32_t), st));
     k_lists_root<<<1, 64, 0, st>>>(t.child, t.nb, t.ni, t.nb_cnt, t.ni_cnt, t.ap_cnt, t.nbr27);
-    for (int d = 1; d <= t.depth; ++d) {
-        const int64_t c0 = t.lvl_off[d], c1 = t.lvl_off[d + 1];
-        k_lists_level<<<ceil_div(c1 - c0, 128), 128, 0, st>>>(c0, c1, t.parent, t.child, t.grid, t.nb, t.ni, t.ap, t.nb_cnt, t.ni_cnt, t.ap_cnt,
-                                                             t.nbr27, c.scal.p + 4);
+    for (int d = 1; d <= t.depth; ++d) {  // children at level d of the internal cells of level d - 1
+        const int64_t a = t.int_off[d - 1], b = t.int_off[d];
+        if (b > a)
+            k_lists_level<<<ceil_div(b - a, 4), 256, 0, st>>>(t.int_ids.p + a, static_cast<int>(b - a), t.child, t.grid, t.nb, t.ni, t.ap, t.nb_cnt,
+                                                              t.ni_cnt, t.ap_cnt, t.nbr27, c.scal.p + 4);
     }
     VG_LAUNCH_CHECK();
 }
