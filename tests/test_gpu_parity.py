@@ -84,6 +84,18 @@ def test_tree_uniform_1m_s64(vg, orc):
     assert_lists_equal(l, rl)
 
 
+def test_tree_deep_cluster(vg, orc):
+    """A cluster 3e-6 wide inside the unit cube needs 19 levels: the
+    build that sorts the first 16 levels' digits must redo the sort in full."""
+    rng = np.random.default_rng(3)
+    x = np.concatenate([rng.random((1000, 3)), 0.3 + 3e-6 * rng.random((400, 3))])
+    t, l = gpu_tree(vg, x, 64)
+    rt, rl = orc.tree_and_lists(x, 64)
+    assert t.level.max() > 16
+    assert_tree_equal(t, rt)
+    assert_lists_equal(l, rl)
+
+
 def test_tree_plummer_16k(vg, orc):
     x, _, _ = orc.problem_initialize(16384, 0, 1e-3)
     t, l = gpu_tree(vg, x, 64)

@@ -111,6 +111,7 @@ struct Ctx {
     DevBuf<uint32_t> val[2];
     DevBuf<uint32_t> rs_hist;
     DevBuf<uint32_t> scal;      // small device scalars (counts, error flags)
+    DevBuf<uint32_t> cmp_cnt;   // per-block counts of the multi-block compaction
     DevBuf<uint32_t> perm;      // sorted position -> caller index
     // particles in sorted order
     DevBuf<double> xs64;        // 4 per particle: x, y, z, m

@@ -203,6 +203,62 @@ __global__ __launch_bounds__(1024) void k_compact(const uint8_t *__restrict__ fl
     if (threadIdx.x == 0) *nout = carry;
 }
 
+// Multi-block ordered compaction: per 1024-flag block counts, an exclusive
+// scan of the counts in one block, then each block writes its entries.
+__global__ __launch_bounds__(1024) void k_flag_counts(const uint8_t *__restrict__ flag, int64_t nflag, uint32_t *__restrict__ bcnt) {
+    __shared__ uint32_t wsum[16];
+    const int64_t i = blockIdx.x * 1024ll + threadIdx.x;
+    const uint64_t bal = __ballot(i < nflag && flag[i]);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = __popcll(bal);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < 16; ++w) t += wsum[w];
+        bcnt[blockIdx.x] = t;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_count_scan(uint32_t *__restrict__ bcnt, int nb, uint32_t *__restrict__ nout) {
+    __shared__ uint32_t carry;
+    __shared__ uint32_t wsum[16];
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    const unsigned lane = threadIdx.x & 63;
+    for (int off = 0; off < nb; off += 1024) {
+        const int i = off + threadIdx.x;
+        const uint32_t v = i < nb ? bcnt[i] : 0u;
+        uint32_t x = v;
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t o = __shfl_up(x, d, 64);
+            if (lane >= static_cast<unsigned>(d)) x += o;
+        }
+        if (lane == 63) wsum[threadIdx.x >> 6] = x;
+        __syncthreads();
+        uint32_t wb = 0;
+        for (unsigned w = 0; w < (threadIdx.x >> 6); ++w) wb += wsum[w];
+        const uint32_t c0 = carry;
+        if (i < nb) bcnt[i] = c0 + wb + x - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry = c0 + wb + x;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *nout = carry;
+}
+
+__global__ __launch_bounds__(1024) void k_flag_write(const uint8_t *__restrict__ flag, int64_t nflag, int64_t base, const uint32_t *__restrict__ boff,
+                                                     int32_t *__restrict__ out) {
+    __shared__ uint32_t wsum[16];
+    const int64_t i = blockIdx.x * 1024ll + threadIdx.x;
+    const unsigned lane = threadIdx.x & 63;
+    const bool f = i < nflag && flag[i];
+    const uint64_t bal = __ballot(f);
+    if (lane == 0) wsum[threadIdx.x >> 6] = __popcll(bal);
+    __syncthreads();
+    uint32_t wb = boff[blockIdx.x];
+    for (unsigned w = 0; w < (threadIdx.x >> 6); ++w) wb += wsum[w];
+    if (f) out[wb + __popcll(bal & ((1ull << lane) - 1))] = static_cast<int32_t>(base + i);
+}
+
 __global__ __launch_bounds__(256) void k_leaf_flags(const int32_t *__restrict__ child, const uint32_t *__restrict__ count, int64_t nc,
                                                     uint8_t *__restrict__ flag) {
     const int64_t c = blockIdx.x * 256ll + threadIdx.x;
@@ -425,6 +481,20 @@ __global__ __launch_bounds__(256) void k_ref_ids(int64_t nc, const int32_t *__re
 
 }  // namespace
 
+// out[0..k) = base + i for the flagged i, in order; k in *nout (device)
+static void compact_flags(Ctx &c, const uint8_t *flag, int64_t n, int64_t base, int32_t *out, uint32_t *nout) {
+    if (n <= 8192) {
+        k_compact<<<1, 1024, 0, c.stream>>>(flag, n, base, out, nout);
+    } else {
+        const int nb = static_cast<int>(ceil_div(n, 1024));
+        c.cmp_cnt.ensure(nb);
+        k_flag_counts<<<nb, 1024, 0, c.stream>>>(flag, n, c.cmp_cnt);
+        k_count_scan<<<1, 1024, 0, c.stream>>>(c.cmp_cnt, nb, nout);
+        k_flag_write<<<nb, 1024, 0, c.stream>>>(flag, n, base, c.cmp_cnt, out);
+    }
+    VG_LAUNCH_CHECK();
+}
+
 void tree_build(Ctx &c) {
     hipStream_t st = c.stream;
     TreeDev &t = c.t;
@@ -437,17 +507,25 @@ void tree_build(Ctx &c) {
     k_aabb<<<nb, 256, 0, st>>>(c.x, n, c.aabb_part);
     k_aabb_final<<<1, 256, 0, st>>>(c.aabb_part, nb, c.box);
     VG_LAUNCH_CHECK();
-    // 2. path codes and stable sort
+    // 2. path codes and a stable sort of the digits of levels 1..SHALLOW
+    //    (bits below the deepest level do not matter: leaves are re-sorted by
+    //    caller index); a tree that needs a deeper level is rebuilt with all
+    //    21 digits sorted.
+    constexpr int SHALLOW = 16;
+    int64_t ncells = 1;
+    uint32_t hbuf[2];
     for (int b = 0; b < 2; ++b) {
         c.key[b].ensure(std::max<int64_t>(1, n));
         c.val[b].ensure(std::max<int64_t>(1, n));
     }
     uint64_t *k0 = c.key[0].p, *k1 = c.key[1].p;
     uint32_t *v0 = c.val[0].p, *v1 = c.val[1].p;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+    const int low_bit = attempt == 0 ? 3 * (MAXD - SHALLOW) : 0;
     if (n) {
         k_codes<<<ceil_div(n, 256), 256, 0, st>>>(c.x, n, c.box, k0, v0);
         VG_LAUNCH_CHECK();
-        radix_sort_pairs(st, k0, v0, k1, v1, n, 0, 3 * MAXD, c.rs_hist);
+        radix_sort_pairs(st, k0, v0, k1, v1, n, low_bit, 3 * MAXD, c.rs_hist);
     }
     // 3. cells, level by level
     t.lvl_off.assign(1, 0);
@@ -470,7 +548,6 @@ void tree_build(Ctx &c) {
     k_root<<<1, 64, 0, st>>>(c.box, n, c.thr, t.start, t.count, t.child, t.parent, t.octant, t.level, t.bmin, t.bmax, t.grid, t.anc, t.int_ids,
                             c.scal.p);
     VG_LAUNCH_CHECK();
-    uint32_t hbuf[2];
     double hbox[8];
     VG_HIP(hipMemcpyAsync(hbox, c.box, 7 * sizeof(double), hipMemcpyDeviceToHost, st));
     VG_HIP(hipMemcpyAsync(hbuf, c.scal.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
@@ -478,13 +555,18 @@ void tree_build(Ctx &c) {
     if (hbox[6] != 0) throw Error(2, "non-finite particle coordinates");
     for (int a = 0; a < 6; ++a) t.box[a] = hbox[a];
     int64_t nint = hbuf[0];
-    int64_t ncells = 1;
+    ncells = 1;
     t.lvl_off.push_back(1);
     t.int_off.push_back(nint);
     int d = 0;
+    bool deeper = false;
     while (nint > 0) {
         if (d >= MAXD)
             throw Error(3, "octree deeper than 21 levels: more than floor(sqrt(8) s) points closer than the root box / 2^21");
+        if (low_bit > 0 && d >= SHALLOW) {  // level d + 1 digits were not sorted
+            deeper = true;
+            break;
+        }
         const int64_t nnew = 8 * nint;
         if (ncells + nnew > cap) {
             while (cap < ncells + nnew) cap *= 2;
@@ -494,8 +576,7 @@ void tree_build(Ctx &c) {
         t.int_ids.ensure(t.int_off.back() + nnew, true, st);
         k_expand<<<ceil_div(nnew, 256), 256, 0, st>>>(d, t.int_ids.p + t.int_off[d], static_cast<int>(nint), ncells, k0, c.thr, t.start, t.count,
                                                        t.child, t.parent, t.octant, t.level, t.bmin, t.bmax, t.grid, t.anc, t.flags);
-        k_compact<<<1, 1024, 0, st>>>(t.flags, nnew, ncells, t.int_ids.p + t.int_off[d + 1], c.scal.p);
-        VG_LAUNCH_CHECK();
+        compact_flags(c, t.flags, nnew, ncells, t.int_ids.p + t.int_off[d + 1], c.scal.p);
         VG_HIP(hipMemcpyAsync(hbuf, c.scal.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
         VG_HIP(hipStreamSynchronize(st));
         ncells += nnew;
@@ -504,13 +585,15 @@ void tree_build(Ctx &c) {
         t.int_off.push_back(t.int_off.back() + nint);
         ++d;
     }
+    if (!deeper) break;
+    }
     t.depth = static_cast<int>(t.lvl_off.size()) - 2;
     t.ncells = ncells;
     // 4. non-empty leaves, leaf order by caller index, particles in sorted order
     t.flags.ensure(ncells, false, st);
     t.leaf_ids.ensure(ncells, false, st);
     k_leaf_flags<<<ceil_div(ncells, 256), 256, 0, st>>>(t.child, t.count, ncells, t.flags);
-    k_compact<<<1, 1024, 0, st>>>(t.flags, ncells, 0, t.leaf_ids, c.scal.p);
+    compact_flags(c, t.flags, ncells, 0, t.leaf_ids, c.scal.p);
     VG_LAUNCH_CHECK();
     VG_HIP(hipMemcpyAsync(hbuf, c.scal.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     VG_HIP(hipStreamSynchronize(st));
