@@ -55,6 +55,9 @@ def main():
     ap.add_argument("--fallback-fraction", type=float, default=0.1,
                     help="fraction of leaves (on the training pairs) whose closure estimate sends them to the order-12 far field")
     ap.add_argument("--learned-err-sample", type=int, default=16384)
+    ap.add_argument("--drift-workload", default="collision16m",
+                    help="workload of the energy-drift comparison (split p=12 against plain FP32 p=12); 'none' skips it")
+    ap.add_argument("--drift-steps", type=int, default=20)
     ap.add_argument("--no-compare-plain", dest="compare_plain", action="store_false",
                     help="skip the plain FP32 p=12 GPU FMM run reported beside the main mode")
     args = ap.parse_args()
@@ -141,6 +144,10 @@ def main():
     if args.learned_workload != "none":
         learned = learned_comparison(vgm, workloads, args)
 
+    drift = None
+    if args.drift_workload != "none" and args.drift_steps > 0:
+        drift = energy_drift(vgm, workloads, args)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(x, w, n)
@@ -156,6 +163,7 @@ def main():
         "cpu_baseline": cpu,
         "vs_plain_gpu": plain,
         "learned": learned,
+        "energy_drift": drift,
         "host_roundtrip_steps_per_s": pcie_steps,
         "phase_ms_per_eval": per_eval,
         "p2p_pairs_per_s": cnt["p2p_pairs"] / (per_eval["p2p"] * 1e-3) if per_eval["p2p"] > 0 else None,
@@ -180,6 +188,27 @@ def plain_reference(vgm, x, v, masses, w, n, steps, value):
     g.close()
     plain = steps / (t1 - t0)
     return {"mode": "plain fp32 p2p + fp32 m2l", "steps_per_s": plain, "speedup": value / plain}
+
+
+def energy_drift(vgm, workloads, args):
+    """Relative total-energy drift |E(T) - E(0)| / |E(0)| over K RK4 steps of the
+    two-Plummer collision, split p = 12 beside plain FP32 p = 12 (FP64 state
+    and energy reductions in both)."""
+    wl = args.drift_workload
+    wdef = workloads.WORKLOADS[wl]
+    x, v, masses = workloads.make(wl, seed=0)
+    n = len(x)
+    out = {"workload": wl, "n": n, "steps": args.drift_steps, "dt": wdef["dt"], "p": wdef["p"]}
+    for name, mode in (("split_p12", "split"), ("plain_fp32_p12", "f32")):
+        with vgm.Gravity(n, wdef["s"], wdef["p"], mode, eps=wdef["eps"], G=1.0, mass=1.0 / n) as g:
+            g.upload(x, v, masses)
+            e0 = sum(g.energy())
+            t0 = time.perf_counter()
+            g.step(wdef["dt"], args.drift_steps)
+            t1 = time.perf_counter()
+            e1 = sum(g.energy())
+        out[name] = {"relative_drift": abs(e1 - e0) / abs(e0), "E0": e0, "E1": e1, "steps_per_s": args.drift_steps / (t1 - t0)}
+    return out
 
 
 def learned_comparison(vgm, workloads, args):
