@@ -569,6 +569,59 @@ struct Bufs<float> {
     }
 };
 
+// One parent per 1024-thread block, for the small levels near the root: the
+// (child q, row r) items are spread over the threads; M2M then adds the 8
+// children's partial rows through LDS.
+template <class R>
+__global__ __launch_bounds__(1024) void k_m2m_wide(const int32_t *__restrict__ ids, const int32_t *__restrict__ child, const uint32_t *__restrict__ count,
+                                                   const R *__restrict__ op, int K, int ldk, R *__restrict__ M) {
+    __shared__ R cm[8 * MAXK];
+    __shared__ R part[8 * MAXK];
+    const int32_t P = ids[blockIdx.x];
+    const int32_t cb = child[P];
+    for (int i = threadIdx.x; i < 8 * K; i += 1024) {
+        const int q = i / K, col = i % K;
+        cm[i] = count[cb + q] ? M[static_cast<size_t>(cb + q) * ldk + col] : R(0);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 8 * K; i += 1024) {
+        const int q = i / K, r = i % K;
+        int j = 0;
+        while ((j + 1) * (j + 1) <= r) ++j;
+        const R *ocol = op + static_cast<size_t>(q) * K * K + r;
+        R sum = 0;
+        for (int col = 0; col < (j + 1) * (j + 1); ++col) sum += ocol[static_cast<size_t>(col) * K] * cm[q * K + col];
+        part[i] = sum;
+    }
+    __syncthreads();
+    for (int r = threadIdx.x; r < K; r += 1024) {
+        R sum = 0;
+        for (int q = 0; q < 8; ++q) sum += part[q * K + r];  // children in order
+        M[static_cast<size_t>(P) * ldk + r] = sum;
+    }
+    for (int i = K + threadIdx.x; i < ldk; i += 1024) M[static_cast<size_t>(P) * ldk + i] = R(0);
+}
+
+template <class R>
+__global__ __launch_bounds__(1024) void k_l2l_wide(const int32_t *__restrict__ ids, const int32_t *__restrict__ child, const uint32_t *__restrict__ count,
+                                                   const R *__restrict__ op, int K, int ldk, R *__restrict__ L) {
+    __shared__ R pl[MAXK];
+    const int32_t P = ids[blockIdx.x];
+    const int32_t cb = child[P];
+    for (int i = threadIdx.x; i < K; i += 1024) pl[i] = L[static_cast<size_t>(P) * ldk + i];
+    __syncthreads();
+    for (int i = threadIdx.x; i < 8 * K; i += 1024) {
+        const int q = i / K, r = i % K;
+        if (!count[cb + q]) continue;
+        int j = 0;
+        while ((j + 1) * (j + 1) <= r) ++j;
+        const R *ocol = op + static_cast<size_t>(8 + q) * K * K + r;
+        R sum = 0;
+        for (int col = j * j; col < K; ++col) sum += ocol[static_cast<size_t>(col) * K] * pl[col];
+        L[static_cast<size_t>(cb + q) * ldk + r] += sum;
+    }
+}
+
 inline int pick_pb(int64_t nparents, int maxpb) {
     int pb = 1;
     while (pb < maxpb && nparents >= static_cast<int64_t>(pb) * 2 * 512) pb *= 2;
@@ -579,6 +632,10 @@ template <class R>
 void m2m_level(Ctx &c, const int32_t *ids, int64_t n) {
     const TreeDev &t = c.t;
     const int pb = pick_pb(n, 8);  // 8 children x PB staged expansions in LDS
+    if (n < 1024) {  // few parents: one wide block each
+        k_m2m_wide<R><<<static_cast<unsigned>(n), 1024, 0, c.stream>>>(ids, t.child, t.count, Bufs<R>::op(c), c.K, c.ldk, Bufs<R>::M(c));
+        return;
+    }
 #define VG_M2M(PB_) \
     k_m2m<R, PB_><<<ceil_div(n, PB_), 256, 0, c.stream>>>(ids, static_cast<int>(n), t.child, t.count, Bufs<R>::op(c), c.K, c.ldk, Bufs<R>::M(c))
     if (pb == 1) VG_M2M(1);
@@ -592,6 +649,10 @@ template <class R>
 void l2l_level(Ctx &c, const int32_t *ids, int64_t n) {
     const TreeDev &t = c.t;
     const int pb = pick_pb(n, 16);
+    if (n < 1024) {
+        k_l2l_wide<R><<<static_cast<unsigned>(n), 1024, 0, c.stream>>>(ids, t.child, t.count, Bufs<R>::op(c), c.K, c.ldk, Bufs<R>::L(c));
+        return;
+    }
 #define VG_L2L(PB_) \
     k_l2l<R, PB_><<<ceil_div(n, PB_), 256, 0, c.stream>>>(ids, static_cast<int>(n), t.child, t.count, Bufs<R>::op(c), c.K, c.ldk, Bufs<R>::L(c))
     if (pb == 1) VG_L2L(1);
