@@ -38,6 +38,7 @@ void swap_expansions(Ctx &c, ExpSet &e) {
     c.M32.swap(e.M32);
     c.L32.swap(e.L32);
     c.opsplit.swap(e.opsplit);
+    c.msplit.swap(e.msplit);
 }
 void fmm_constants_init();
 
@@ -323,8 +324,8 @@ int vipp_grav_create(const vipp_grav_config *cfg, vipp_grav_ctx **out) {
             c.thr = static_cast<uint32_t>(thr);
             c.p = static_cast<int>(cfg->p);
             c.K = (c.p + 1) * (c.p + 1);
-            c.ldk = c.prec == 2 ? (c.K + 15) / 16 * 16 : c.K;
             c.prec = static_cast<int>(cfg->precision);
+            c.ldk = c.prec == 2 ? (c.K + 15) / 16 * 16 : c.K;  // the MFMA path reads whole 16-coefficient k-steps
             c.eps = cfg->epsilon;
             c.G = cfg->G;
             c.mass = cfg->mass;

@@ -50,7 +50,7 @@ struct ExpSet {
     int p = 0, K = 0, ldk = 0;
     DevBuf<double> harm, op64;
     DevBuf<float> op32, M32, L32;
-    DevBuf<uint16_t> opsplit;
+    DevBuf<uint16_t> opsplit, msplit;
 };
 
 // learned closure state (closure.hip)
@@ -129,6 +129,7 @@ struct Ctx {
     DevBuf<double> op64;        // M2M (8), L2L (8), M2L (343) blocks of K*K
     DevBuf<float> op32;
     DevBuf<uint16_t> opsplit;   // M2L operators as BF16 hi/lo MFMA fragments
+    DevBuf<uint16_t> msplit;    // multipoles as BF16 hi / lo halves (per cell ldk hi, then ldk lo)
     DevBuf<double> consts;      // A_n^m and recurrence constants (device copy)
     DevBuf<double> opvec;       // operator offset vectors (device copy)
     std::vector<double> opvec_host;

@@ -72,7 +72,7 @@ __global__ __launch_bounds__(256) void k_split_ops(const double *__restrict__ op
 template <int P, int NT, int NG>
 __global__ __launch_bounds__(64 * NG * mt_of((P + 1) * (P + 1))) void k_m2l_split(const int32_t *__restrict__ parents, int nparents,
                                                                              const int32_t *__restrict__ child, const uint32_t *__restrict__ count,
-                                                                             const int32_t *__restrict__ nbr27, const float *__restrict__ M,
+                                                                             const int32_t *__restrict__ nbr27, const uint4 *__restrict__ Ms,
                                                                              float *__restrict__ L, const bf16x8 *__restrict__ ops, int ldk) {
     constexpr int K = (P + 1) * (P + 1);
     constexpr int MT = mt_of(K), NKS = ks_of(K);
@@ -88,6 +88,7 @@ __global__ __launch_bounds__(64 * NG * mt_of((P + 1) * (P + 1))) void k_m2l_spli
     __shared__ int32_t s_sb[27][TB];         // first child of each parent neighbour (-1: none)
     __shared__ uint32_t s_live[7][TB];       // bit 8 dn + qs: that source cell holds particles
     __shared__ float s_sgn[16 * NKS];        // D_qt: sign of each coefficient under this octant's mirror
+    __shared__ uint4 s_msk[2 * NKS];         // the same as BF16 sign-bit masks, 8 coefficients per entry
     // XCD-aware placement: blocks go round-robin to the 8 XCDs, so XCD x takes
     // the contiguous run of work items [x per, (x+1) per); the 8 octant blocks
     // of a parent chunk, which read the same source multipoles and operators,
@@ -131,6 +132,13 @@ __global__ __launch_bounds__(64 * NG * mt_of((P + 1) * (P + 1))) void k_m2l_spli
         if ((qt & 4) && ((nn + m) & 1)) sg = -sg;
         s_sgn[k] = static_cast<float>(sg);
     }
+    __syncthreads();
+    for (int k8 = tid; k8 < 2 * NKS; k8 += NTH) {
+        uint32_t w[4];
+        for (int i = 0; i < 4; ++i)
+            w[i] = (s_sgn[8 * k8 + 2 * i] < 0 ? 0x8000u : 0u) | (s_sgn[8 * k8 + 2 * i + 1] < 0 ? 0x80000000u : 0u);
+        s_msk[k8] = make_uint4(w[0], w[1], w[2], w[3]);
+    }
     for (int t = tid; t < TB; t += NTH) {
         const int pi = chunk * TB + t;
         int32_t P0 = pi < nparents ? parents[pi] : -1;
@@ -151,8 +159,10 @@ __global__ __launch_bounds__(64 * NG * mt_of((P + 1) * (P + 1))) void k_m2l_spli
         const int dn = s_dn[st], bit = 8 * dn + s_qs[st];
         return ((s_live[bit >> 5][t] >> (bit & 31)) & 1u) ? s_sb[dn][t] + s_qs[st] : -1;
     };
-    // gather the B operand of step st into registers: chunk c -> target c / (2 NKS), 8 floats at k = 8 (c % (2 NKS))
-    float bq[CPT][8];
+    // gather the B operand of step st into registers: chunk c -> target c / (2 NKS),
+    // the 8 coefficients at k = 8 (c % (2 NKS)) as BF16 hi and lo halves (Ms:
+    // per cell ldk hi values then ldk lo values, 16 bytes per 8 coefficients)
+    uint4 bh[CPT], bl[CPT];
     auto gather = [&](int st) {
 #pragma unroll
         for (int u = 0; u < CPT; ++u) {
@@ -160,35 +170,30 @@ __global__ __launch_bounds__(64 * NG * mt_of((P + 1) * (P + 1))) void k_m2l_spli
             const int t = c / (2 * NKS), k8 = c % (2 * NKS);
             const int32_t s = (c < CH) ? source_of(st, t) : -1;
             if (s >= 0) {
-                const float *src = M + static_cast<int64_t>(s) * ldk + 8 * k8;
-                const float4 a = *reinterpret_cast<const float4 *>(src);
-                const float4 b = *reinterpret_cast<const float4 *>(src + 4);
-                bq[u][0] = a.x; bq[u][1] = a.y; bq[u][2] = a.z; bq[u][3] = a.w;
-                bq[u][4] = b.x; bq[u][5] = b.y; bq[u][6] = b.z; bq[u][7] = b.w;
+                const uint4 *src = Ms + static_cast<int64_t>(s) * (ldk / 4) + k8;  // ldk / 8 chunks of hi, then of lo
+                bh[u] = src[0];
+                bl[u] = src[ldk / 8];
             } else {
-#pragma unroll
-                for (int e = 0; e < 8; ++e) bq[u][e] = 0.0f;
+                bh[u] = make_uint4(0, 0, 0, 0);
+                bl[u] = make_uint4(0, 0, 0, 0);
             }
         }
     };
-    // split and store as MFMA B fragments: lane (k8 % 2) * 32 + t % 32 of n-tile t / 32, k-step k8 / 2
+    // store as MFMA B fragments with this octant's signs (negation commutes with
+    // the split, so it is a flip of the sign bits): lane (k8 % 2) * 32 + t % 32 of
+    // n-tile t / 32, k-step k8 / 2
     auto stash = [&]() {
 #pragma unroll
         for (int u = 0; u < CPT; ++u) {
             const int c = tid + u * NTH;
             if (c >= CH) continue;
             const int t = c / (2 * NKS), k8 = c % (2 * NKS);
-            bf16x8 h, l;
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const float b = bq[u][e] * s_sgn[8 * k8 + e];
-                const __bf16 hb = static_cast<__bf16>(b);
-                h[e] = hb;
-                l[e] = static_cast<__bf16>(b - static_cast<float>(hb));
-            }
+            const uint4 m = s_msk[k8];
+            const uint4 h = make_uint4(bh[u].x ^ m.x, bh[u].y ^ m.y, bh[u].z ^ m.z, bh[u].w ^ m.w);
+            const uint4 l = make_uint4(bl[u].x ^ m.x, bl[u].y ^ m.y, bl[u].z ^ m.z, bl[u].w ^ m.w);
             const int ln = ((k8 & 1) * 32 + (t & 31)) ^ (k8 & 15);
-            sB[k8 >> 1][t >> 5][0][ln] = h;
-            sB[k8 >> 1][t >> 5][1][ln] = l;
+            sB[k8 >> 1][t >> 5][0][ln] = __builtin_bit_cast(bf16x8, h);  // typed stores: the loads below are bf16x8
+            sB[k8 >> 1][t >> 5][1][ln] = __builtin_bit_cast(bf16x8, l);
         }
     };
     f32x16 acc[NW];
@@ -254,13 +259,26 @@ __global__ __launch_bounds__(64 * NG * mt_of((P + 1) * (P + 1))) void k_m2l_spli
     }
 }
 
+// M (FP32, row stride ldk) -> per cell ldk BF16 hi values then ldk lo values
+__global__ __launch_bounds__(256) void k_split_m(const float *__restrict__ M, int64_t n, int ldk, uint16_t *__restrict__ Ms) {
+    const int64_t i = blockIdx.x * 256ll + threadIdx.x;
+    if (i >= n * ldk) return;
+    const int64_t c = i / ldk, k = i % ldk;
+    const float x = M[i];
+    const __bf16 h = static_cast<__bf16>(x);
+    const __bf16 l = static_cast<__bf16>(x - static_cast<float>(h));
+    Ms[c * 2 * ldk + k] = __builtin_bit_cast(uint16_t, h);
+    Ms[c * 2 * ldk + ldk + k] = __builtin_bit_cast(uint16_t, l);
+}
+
 template <int P>
 void launch_split(Ctx &c, const int32_t *parents, int np) {
     constexpr int K = (P + 1) * (P + 1);
     constexpr int NT = 4, NG = 2;  // 128 targets, 12 wavefronts: 3 per SIMD
     const unsigned grid = 8 * ceil_div(np, 32 * NT);  // one block per (parent chunk, child octant)
-    k_m2l_split<P, NT, NG><<<grid, 64 * NG * mt_of(K), 0, c.stream>>>(parents, np, c.t.child, c.t.count, c.t.nbr27, c.M32, c.L32,
-                                                             reinterpret_cast<const bf16x8 *>(c.opsplit.p), c.ldk);
+    k_m2l_split<P, NT, NG><<<grid, 64 * NG * mt_of(K), 0, c.stream>>>(parents, np, c.t.child, c.t.count, c.t.nbr27,
+                                                                     reinterpret_cast<const uint4 *>(c.msplit.p), c.L32,
+                                                                     reinterpret_cast<const bf16x8 *>(c.opsplit.p), c.ldk);
     VG_LAUNCH_CHECK();
 }
 
@@ -277,6 +295,10 @@ void m2l_split_operators(Ctx &c, int nvec) {
 
 void fmm_m2l_split_list(Ctx &c, const int32_t *parents, int np) {
     if (np <= 0) return;
+    const int64_t nc = c.t.ncells;
+    c.msplit.ensure(static_cast<size_t>(nc) * 2 * c.ldk);
+    k_split_m<<<ceil_div(nc * c.ldk, 256), 256, 0, c.stream>>>(c.M32, nc, c.ldk, c.msplit);
+    VG_LAUNCH_CHECK();
     switch (c.p) {
         case 1: launch_split<1>(c, parents, np); break;
         case 2: launch_split<2>(c, parents, np); break;
