@@ -283,3 +283,22 @@ def test_deterministic(vg, orc):
             gr.upload(x)
             outs.append(gr.accelerations())
     assert np.array_equal(outs[0], outs[1])
+
+
+# ------------------------------------------------------------- every order --
+@pytest.mark.parametrize("p", list(range(1, 13)))
+def test_every_order(vg, orc, p):
+    """Each order p is its own instantiation of L2P and of the split M2L: FP64
+    against the oracle (1e-11 of max |a|), FP32 and split against the FP64
+    result of the same tree (median 1e-5: FP32 rounding, BF16 hi/lo split)."""
+    x, _, _ = orc.problem_initialize(2000, 7, 1e-3)
+    ref = orc.fmm(x, 16, p, eps=1e-3)
+    out = {}
+    for mode in ("f64", "f32", "split"):
+        with vg.Gravity(len(x), 16, p, mode, eps=1e-3) as gr:
+            gr.upload(x)
+            out[mode] = gr.accelerations()
+    assert np.abs(out["f64"] - ref).max() <= 1e-11 * np.abs(ref).max()
+    for mode in ("f32", "split"):
+        rel = np.linalg.norm(out[mode] - out["f64"], axis=1) / np.linalg.norm(out["f64"], axis=1)
+        assert np.median(rel) < 1e-5, (mode, np.median(rel))

@@ -24,6 +24,7 @@
 #include "gravity.hpp"
 
 #include <cmath>
+#include <mutex>
 
 namespace vg {
 namespace {
@@ -751,9 +752,15 @@ void downward_t(Ctx &c) {
 
 void m2l_split_operators(Ctx &c, int nvec);
 
+// The tables live in each device's constant memory: upload them once per
+// device (the caller has made the context's device current).
 void fmm_constants_init() {
-    static bool done = false;
-    if (done) return;
+    static std::mutex mu;
+    static std::vector<bool> done;
+    int dev = 0;
+    VG_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    if (static_cast<size_t>(dev) < done.size() && done[static_cast<size_t>(dev)]) return;
     std::vector<double> A((AL + 1) * AW, 0.0);
     for (int n = 0; n <= AL; ++n)
         for (int m = -n; m <= n; ++m) {
@@ -788,7 +795,8 @@ void fmm_constants_init() {
     VG_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cLz), lz.data(), lz.size() * sizeof(double)));
     VG_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cLa), la.data(), la.size() * sizeof(double)));
     VG_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cLb), lb.data(), lb.size() * sizeof(double)));
-    done = true;
+    if (done.size() <= static_cast<size_t>(dev)) done.resize(static_cast<size_t>(dev) + 1, false);
+    done[static_cast<size_t>(dev)] = true;
 }
 
 void operators_build(Ctx &c) {
