@@ -24,7 +24,7 @@ namespace {
 
 constexpr int P2P_THREADS = 256;
 constexpr int MAXE = 27 * (MAXD + 1);
-constexpr uint32_t P2P_CHUNK = 4096;  // sources per work item
+constexpr uint32_t P2P_CHUNK = 8192;  // sources per work item
 
 template <class R>
 struct Vec4;

@@ -3,7 +3,8 @@
 // Three kernels per 8-bit digit: a tile histogram, a per-digit scan over the
 // tiles, and a scatter that ranks equal digits inside each 64-lane wavefront
 // with ballots, so the order of equal keys is kept (the tree needs points of a
-// cell in the caller's index order).
+// cell in the caller's index order).  The scatter reorders each tile in LDS
+// before writing, so its global writes are runs of consecutive addresses.
 #include "gravity.hpp"
 
 namespace vg {
@@ -69,28 +70,57 @@ __global__ __launch_bounds__(1024) void k_rs_scan(uint32_t *__restrict__ hist, c
     }
 }
 
+// Scatter of one 4096-key tile: the keys are first ranked stably by digit
+// (ballots inside a wavefront, running counts across wavefronts and rounds)
+// and reordered in LDS, then written out in that order, so consecutive threads
+// write consecutive addresses of each digit's run.
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *__restrict__ key, const uint32_t *__restrict__ val, uint64_t *__restrict__ key_out,
                                                            uint32_t *__restrict__ val_out, int64_t n, int shift, const uint32_t *__restrict__ hist,
                                                            unsigned ntiles) {
-    __shared__ uint32_t run[RS_BINS];
+    __shared__ uint64_t sk[RS_TILE];
+    __shared__ uint32_t sv[RS_TILE];
+    __shared__ uint32_t run[RS_BINS];    // next tile-local slot of each digit
+    __shared__ uint32_t start[RS_BINS];  // tile-local start of each digit
     __shared__ uint32_t wcnt[RS_WAVES][RS_BINS];
+    __shared__ uint32_t wsum[RS_WAVES];
     const unsigned tile = blockIdx.x;
     const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    run[threadIdx.x] = hist[static_cast<size_t>(threadIdx.x) * ntiles + tile];
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const int64_t base = static_cast<int64_t>(tile) * RS_TILE;
+    const int nt = static_cast<int>(min<int64_t>(RS_TILE, n - base));
+    // tile histogram and its exclusive scan
+    run[threadIdx.x] = 0;
+    __syncthreads();
+    for (int j = threadIdx.x; j < nt; j += RS_THREADS) atomicAdd(&run[(key[base + j] >> shift) & 0xFF], 1u);
+    __syncthreads();
+    {
+        const uint32_t c = run[threadIdx.x];
+        uint32_t x = c;
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t o = __shfl_up(x, d, 64);
+            if (lane >= static_cast<unsigned>(d)) x += o;
+        }
+        if (lane == 63) wsum[wave] = x;
+        __syncthreads();
+        uint32_t wb = 0;
+        for (unsigned w = 0; w < wave; ++w) wb += wsum[w];
+        start[threadIdx.x] = wb + x - c;
+        run[threadIdx.x] = wb + x - c;
+    }
+    __syncthreads();
+    // stable local ranks, written to LDS in digit order
     for (int r = 0; r < RS_ITEMS; ++r) {
 #pragma unroll
         for (int w = 0; w < RS_WAVES; ++w) wcnt[w][threadIdx.x] = 0;
         __syncthreads();
-        const int64_t i = base + r * RS_THREADS + threadIdx.x;
-        const bool ok = i < n;
+        const int j = r * RS_THREADS + threadIdx.x;
+        const bool ok = j < nt;
         uint64_t k = 0;
         uint32_t v = 0;
         unsigned d = 0;
         if (ok) {
-            k = key[i];
-            v = val[i];
+            k = key[base + j];
+            v = val[base + j];
             d = static_cast<unsigned>((k >> shift) & 0xFF);
         }
         uint64_t match = __ballot(ok);
@@ -107,8 +137,8 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *__res
         if (ok) {
             uint32_t pos = run[d] + rank;
             for (unsigned w = 0; w < wave; ++w) pos += wcnt[w][d];
-            key_out[pos] = k;
-            val_out[pos] = v;
+            sk[pos] = k;
+            sv[pos] = v;
         }
         __syncthreads();
         uint32_t add = 0;
@@ -116,6 +146,14 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *__res
         for (int w = 0; w < RS_WAVES; ++w) add += wcnt[w][threadIdx.x];
         run[threadIdx.x] += add;
         __syncthreads();
+    }
+    // write out in LDS order: slot j of digit d goes to its global run
+    for (int j = threadIdx.x; j < nt; j += RS_THREADS) {
+        const uint64_t k = sk[j];
+        const unsigned d = static_cast<unsigned>((k >> shift) & 0xFF);
+        const uint32_t pos = hist[static_cast<size_t>(d) * ntiles + tile] + (j - start[d]);
+        key_out[pos] = k;
+        val_out[pos] = sv[j];
     }
 }
 
