@@ -64,11 +64,13 @@ __global__ __launch_bounds__(256) void k_split_ops(const double *__restrict__ op
 // targets.  Per offset step:
 //   A (the wavefront's m-tile of T_o, hi and lo) streams from L2 straight
 //     into registers, two k-steps ahead;
-//   B (the targets' source multipoles) sits in LDS as hi/lo MFMA fragments,
-//     gathered for the next step into registers while this step computes.
-// The B fragment of lane l is stored in slot l ^ (k8 & 15), k8 = 2 ks + l / 32,
-// so that both the row-wise stores and the fragment loads spread over the
-// LDS banks.
+//   B (the targets' source multipoles) sits in LDS as hi/lo MFMA fragments.
+// The k-steps of an offset are cut into two halves, and B is double-buffered
+// by half: while a half is multiplied from one buffer, the next half is
+// gathered into registers and stored into the other, so the only pause is one
+// barrier per half.  The B fragment of lane l is stored in slot
+// l ^ (k8 & 15), k8 = 2 ks + l / 32 (ks counted within the half), so that
+// both the row-wise stores and the fragment loads spread over the LDS banks.
 template <int P, int NT, int NG>
 __global__ __launch_bounds__(64 * NG * mt_of((P + 1) * (P + 1))) void k_m2l_split(const int32_t *__restrict__ parents, int nparents,
                                                                              const int32_t *__restrict__ child, const uint32_t *__restrict__ count,
@@ -81,9 +83,10 @@ __global__ __launch_bounds__(64 * NG * mt_of((P + 1) * (P + 1))) void k_m2l_spli
     static_assert(NT % NG == 0, "n-tiles must split evenly over the wavefronts of an m-tile");
     constexpr int TB = 32 * NT;              // targets per block
     constexpr int TILE = 2 * NKS * 64;       // bf16x8 vectors of one m-tile of an operator (hi and lo)
-    constexpr int CH = TB * 2 * NKS;         // 8-float chunks of the B operand per step
-    constexpr int CPT = (CH + NTH - 1) / NTH;
-    __shared__ bf16x8 sB[NKS][NT][2][64];    // [k-step][n-tile][hi/lo][lane]
+    constexpr int NH = NKS >= 2 ? 2 : 1;      // halves per offset step
+    constexpr int KH0 = (NKS + 1) / 2;        // k-steps of the first half (NH = 1: all)
+    constexpr int CPT = (TB * 2 * KH0 + NTH - 1) / NTH;  // 8-coefficient chunks per thread and half
+    __shared__ bf16x8 sB[2][KH0][NT][2][64];  // [buffer][k-step in half][n-tile][hi/lo][lane]
     __shared__ int16_t s_dn[189], s_qs[189], s_o[189];
     __shared__ int32_t s_sb[27][TB];         // first child of each parent neighbour (-1: none)
     __shared__ uint32_t s_live[7][TB];       // bit 8 dn + qs: that source cell holds particles
@@ -97,7 +100,9 @@ __global__ __launch_bounds__(64 * NG * mt_of((P + 1) * (P + 1))) void k_m2l_spli
     const int w = (blockIdx.x & 7) * nchunks + (blockIdx.x >> 3);
     if (w >= 8 * nchunks) return;
     const int chunk = w >> 3, qt = w & 7;
-    const int tid = threadIdx.x, lane = tid & 63, mt = (tid >> 6) % MT, n0 = ((tid >> 6) / MT) * NW;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform, so addresses built from it stay scalar
+    const int mt = wid % MT, n0 = (wid / MT) * NW;
     // Every block walks the 189 offsets e of octant 0 in one fixed order.  For
     // octant qt the geometry is octant 0's mirrored in the axes where qt has a
     // bit set (parent neighbour d -> mirror(d), source octant q -> q ^ qt), and
@@ -155,45 +160,50 @@ __global__ __launch_bounds__(64 * NG * mt_of((P + 1) * (P + 1))) void k_m2l_spli
         for (int w = 0; w < 7; ++w) s_live[w][t] = live[w];
     }
     __syncthreads();
+    // source cell of target t at step st (-1: none or empty); st is uniform
     auto source_of = [&](int st, int t) -> int32_t {
-        const int dn = s_dn[st], bit = 8 * dn + s_qs[st];
-        return ((s_live[bit >> 5][t] >> (bit & 31)) & 1u) ? s_sb[dn][t] + s_qs[st] : -1;
+        const int dn = __builtin_amdgcn_readfirstlane(s_dn[st]), qs = __builtin_amdgcn_readfirstlane(s_qs[st]);
+        const int bit = 8 * dn + qs;
+        return ((s_live[bit >> 5][t] >> (bit & 31)) & 1u) ? s_sb[dn][t] + qs : -1;
     };
-    // gather the B operand of step st into registers: chunk c -> target c / (2 NKS),
-    // the 8 coefficients at k = 8 (c % (2 NKS)) as BF16 hi and lo halves (Ms:
-    // per cell ldk hi values then ldk lo values, 16 bytes per 8 coefficients)
-    uint4 bh[CPT], bl[CPT];
-    auto gather = [&](int st) {
+    // gather half h of step st's B operand into registers: chunk c -> target
+    // c / (2 kh), the 8 coefficients at k = 8 k8, k8 = 2 KH0 h + c % (2 kh), as
+    // BF16 hi and lo halves (Ms: per cell ldk hi values then ldk lo values, 16
+    // bytes per 8 coefficients)
+    uint4 gh[CPT], gl[CPT];
+    auto gather = [&](int st, int h) {
+        const int kh = h ? NKS - KH0 : KH0;
 #pragma unroll
         for (int u = 0; u < CPT; ++u) {
             const int c = tid + u * NTH;
-            const int t = c / (2 * NKS), k8 = c % (2 * NKS);
-            const int32_t s = (c < CH) ? source_of(st, t) : -1;
-            if (s >= 0) {
-                const uint4 *src = Ms + static_cast<int64_t>(s) * (ldk / 4) + k8;  // ldk / 8 chunks of hi, then of lo
-                bh[u] = src[0];
-                bl[u] = src[ldk / 8];
+            const int t = c / (2 * kh), k8 = 2 * KH0 * h + c % (2 * kh);
+            const int32_t src_cell = (c < TB * 2 * kh) ? source_of(st, t) : -1;
+            if (src_cell >= 0) {
+                const uint4 *src = Ms + static_cast<int64_t>(src_cell) * (ldk / 4) + k8;  // ldk / 8 chunks of hi, then of lo
+                gh[u] = src[0];
+                gl[u] = src[ldk / 8];
             } else {
-                bh[u] = make_uint4(0, 0, 0, 0);
-                bl[u] = make_uint4(0, 0, 0, 0);
+                gh[u] = make_uint4(0, 0, 0, 0);
+                gl[u] = make_uint4(0, 0, 0, 0);
             }
         }
     };
-    // store as MFMA B fragments with this octant's signs (negation commutes with
-    // the split, so it is a flip of the sign bits): lane (k8 % 2) * 32 + t % 32 of
-    // n-tile t / 32, k-step k8 / 2
-    auto stash = [&]() {
+    // store into buffer bf as MFMA B fragments with this octant's signs
+    // (negation commutes with the split, so it is a flip of the sign bits):
+    // lane (k8l % 2) * 32 + t % 32 of n-tile t / 32, k-step k8l / 2 of the half
+    auto stash = [&](int bf, int h) {
+        const int kh = h ? NKS - KH0 : KH0;
 #pragma unroll
         for (int u = 0; u < CPT; ++u) {
             const int c = tid + u * NTH;
-            if (c >= CH) continue;
-            const int t = c / (2 * NKS), k8 = c % (2 * NKS);
-            const uint4 m = s_msk[k8];
-            const uint4 h = make_uint4(bh[u].x ^ m.x, bh[u].y ^ m.y, bh[u].z ^ m.z, bh[u].w ^ m.w);
-            const uint4 l = make_uint4(bl[u].x ^ m.x, bl[u].y ^ m.y, bl[u].z ^ m.z, bl[u].w ^ m.w);
-            const int ln = ((k8 & 1) * 32 + (t & 31)) ^ (k8 & 15);
-            sB[k8 >> 1][t >> 5][0][ln] = __builtin_bit_cast(bf16x8, h);  // typed stores: the loads below are bf16x8
-            sB[k8 >> 1][t >> 5][1][ln] = __builtin_bit_cast(bf16x8, l);
+            if (c >= TB * 2 * kh) continue;
+            const int t = c / (2 * kh), k8l = c % (2 * kh);
+            const uint4 m = s_msk[2 * KH0 * h + k8l];
+            const uint4 hv = make_uint4(gh[u].x ^ m.x, gh[u].y ^ m.y, gh[u].z ^ m.z, gh[u].w ^ m.w);
+            const uint4 lv = make_uint4(gl[u].x ^ m.x, gl[u].y ^ m.y, gl[u].z ^ m.z, gl[u].w ^ m.w);
+            const int ln = ((k8l & 1) * 32 + (t & 31)) ^ (k8l & 15);
+            sB[bf][k8l >> 1][t >> 5][0][ln] = __builtin_bit_cast(bf16x8, hv);  // typed stores: the loads below are bf16x8
+            sB[bf][k8l >> 1][t >> 5][1][ln] = __builtin_bit_cast(bf16x8, lv);
         }
     };
     f32x16 acc[NW];
@@ -201,46 +211,92 @@ __global__ __launch_bounds__(64 * NG * mt_of((P + 1) * (P + 1))) void k_m2l_spli
     for (int n = 0; n < NW; ++n)
 #pragma unroll
         for (int r = 0; r < 16; ++r) acc[n][r] = 0.0f;
-    gather(0);
-    stash();
+    gather(0, 0);
+    stash(0, 0);
     __syncthreads();
-    // A fragments of this wavefront's m-tile: flat index q = step * NKS + ks, prefetched two ahead
+    // A fragments of this wavefront's m-tile for operator o (uniform), flat k-step q
     constexpr int TOT = 189 * NKS;
-    auto afrag = [&](int q, int h) -> bf16x8 {
-        const int st = q / NKS, ks = q % NKS;
-        return ops[(static_cast<int64_t>(s_o[st]) * MT + mt) * TILE + h * NKS * 64 + ks * 64 + lane];
+    constexpr int PF = 3;  // A prefetch distance in k-steps
+    auto afrag = [&](int o, int ks, int h) -> bf16x8 {
+        const bf16x8 *base = ops + (static_cast<int64_t>(o) * MT + mt) * TILE + h * NKS * 64 + ks * 64;
+        return base[lane];
     };
-    bf16x8 a0h = afrag(0, 0), a0l = afrag(0, 1);
-    bf16x8 a1h = afrag(1, 0), a1l = afrag(1, 1);
+    // B fragments of k-step ksl of buffer bf for this wavefront's n-tiles
+    // (slot = lane ^ ((2 ksl + lane / 32) % 16) = (lane ^ lane / 32) ^ (2 ksl % 16):
+    // one XOR with a constant on a per-lane byte offset)
+    uint32_t lb = (lane ^ (lane >> 5)) * 16u;
+    auto bfrag = [&](int bf, int ksl, bf16x8 *h, bf16x8 *l) {
+        const uint32_t off = lb ^ (((2u * ksl) & 14u) << 4);
+        const char *row = reinterpret_cast<const char *>(&sB[bf][ksl][n0][0][0]) + off;
+#pragma unroll
+        for (int n = 0; n < NW; ++n) {
+            h[n] = *reinterpret_cast<const bf16x8 *>(row + (2 * n) * 1024);
+            l[n] = *reinterpret_cast<const bf16x8 *>(row + (2 * n + 1) * 1024);
+        }
+    };
+    // Software pipeline, kept in place by scheduling barriers: during k-step q
+    // the A fragments of q + 1 .. q + PF and the B fragments of q + 1 are in
+    // flight.  The
+    // next half is gathered at the first k-step of a half, after that k-step's
+    // A loads, and stored at the fourth (vector loads complete in issue order,
+    // so the A loads issued after the gather already wait for it there).
+    // operator of flat k-step q: step q / NKS, k-step q % NKS
+    auto op_of = [&](int q) -> int { return __builtin_amdgcn_readfirstlane(s_o[q / NKS]); };
+    bf16x8 ah[PF], al[PF];  // A of k-steps q .. q + PF - 1
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+        const int o = op_of(i);
+        ah[i] = afrag(o, i % NKS, 0);
+        al[i] = afrag(o, i % NKS, 1);
+    }
+    bf16x8 bh[NW], bl[NW];
+    bfrag(0, 0, bh, bl);
     for (int st = 0; st < 189; ++st) {
-        if (st + 1 < 189) gather(st + 1);  // in flight during this step
+        asm volatile("" : "+v"(lb));  // recompute the B offsets per step rather than hold them all
 #pragma unroll
         for (int ks = 0; ks < NKS; ++ks) {
+            const int h = ks < KH0 ? 0 : 1;             // half of this k-step
+            const int ksl = ks - h * KH0, kh = h ? NKS - KH0 : KH0;
+            const int bf = NH == 2 ? h : (st & 1);      // its buffer
+            const int nst = (NH == 2 && h == 0) ? st : st + 1, nhf = (NH == 2 && h == 0) ? 1 : 0;  // the next half
+            const int nbf = NH == 2 ? nhf : ((st + 1) & 1);
+            const bool has_next = nst < 189;
             const int q = st * NKS + ks;
-            const bf16x8 ch = a0h, cl = a0l;
-            a0h = a1h;
-            a0l = a1l;
-            if (q + 2 < TOT) {
-                a1h = afrag(q + 2, 0);
-                a1l = afrag(q + 2, 1);
-            }
-            const int slot = lane ^ ((2 * ks + (lane >> 5)) & 15);
-            bf16x8 bh[NW], bl[NW];
+            const bf16x8 ch = ah[0], cl = al[0];
 #pragma unroll
-            for (int n = 0; n < NW; ++n) {
-                bh[n] = sB[ks][n0 + n][0][slot];
-                bl[n] = sB[ks][n0 + n][1][slot];
+            for (int i = 0; i + 1 < PF; ++i) {
+                ah[i] = ah[i + 1];
+                al[i] = al[i + 1];
             }
+            if (q + PF < TOT) {
+                const int o = op_of(q + PF);
+                ah[PF - 1] = afrag(o, (ks + PF) % NKS, 0);
+                al[PF - 1] = afrag(o, (ks + PF) % NKS, 1);
+            }
+            if (ksl == 0 && has_next) gather(nst, nhf);
+            bf16x8 nh[NW], nl[NW];
+            if (ksl + 1 < kh) bfrag(bf, ksl + 1, nh, nl);
+            if (ksl == (kh - 1 < 3 ? kh - 1 : 3) && has_next) stash(nbf, nhf);
+            if (ksl + 1 == kh) {
+                // this half's B is in registers and the next half is stored:
+                // one barrier, then the first fragments of the next half
+                __syncthreads();
+                if (has_next) bfrag(nbf, 0, nh, nl);
+            }
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int n = 0; n < NW; ++n) acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ch, bh[n], acc[n], 0, 0, 0);
 #pragma unroll
             for (int n = 0; n < NW; ++n) acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ch, bl[n], acc[n], 0, 0, 0);
 #pragma unroll
             for (int n = 0; n < NW; ++n) acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(cl, bh[n], acc[n], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int n = 0; n < NW; ++n) {
+                bh[n] = nh[n];
+                bl[n] = nl[n];
+            }
         }
-        __syncthreads();  // every wavefront is done with this step's B
-        if (st + 1 < 189) stash();
-        __syncthreads();
     }
     // C[8(r/4) + 4(l/32) + r%4][l%32] of n-tile n -> row of this m-tile, target 32 n + l%32
 #pragma unroll
