@@ -60,10 +60,13 @@ __global__ __launch_bounds__(256) void k_split_ops(const double *__restrict__ op
 }
 
 // One block: 32*NT targets (children of octant qt of 32*NT parents) and NG
-// wavefronts per 32-row m-tile of the output, each owning NT/NG n-tiles of 32
-// targets.  Per offset step:
+// wavefronts per 32-row m-tile of the output.  Without KS each of them owns
+// NT/NG n-tiles of 32 targets over all k-steps; with KS (NG = 2) both own all
+// NT n-tiles, one the even and one the odd k-steps, and their sums are added
+// at the end, so every A fragment is loaded by one wavefront only.  Per offset
+// step:
 //   A (the wavefront's m-tile of T_o, hi and lo) streams from L2 straight
-//     into registers, two k-steps ahead;
+//     into registers, a few k-steps ahead;
 //   B (the targets' source multipoles) sits in LDS as hi/lo MFMA fragments.
 // The k-steps of an offset are cut into two halves, and B is double-buffered
 // by half: while a half is multiplied from one buffer, the next half is
@@ -71,7 +74,7 @@ __global__ __launch_bounds__(256) void k_split_ops(const double *__restrict__ op
 // barrier per half.  The B fragment of lane l is stored in slot
 // l ^ (k8 & 15), k8 = 2 ks + l / 32 (ks counted within the half), so that
 // both the row-wise stores and the fragment loads spread over the LDS banks.
-template <int P, int NT, int NG>
+template <int P, int NT, int NG, bool KS>
 __global__ __launch_bounds__(64 * NG * mt_of((P + 1) * (P + 1))) void k_m2l_split(const int32_t *__restrict__ parents, int nparents,
                                                                              const int32_t *__restrict__ child, const uint32_t *__restrict__ count,
                                                                              const int32_t *__restrict__ nbr27, const uint4 *__restrict__ Ms,
@@ -79,12 +82,13 @@ __global__ __launch_bounds__(64 * NG * mt_of((P + 1) * (P + 1))) void k_m2l_spli
     constexpr int K = (P + 1) * (P + 1);
     constexpr int MT = mt_of(K), NKS = ks_of(K);
     constexpr int NTH = 64 * MT * NG;        // threads
-    constexpr int NW = NT / NG;              // n-tiles per wavefront
+    constexpr int NW = KS ? NT : NT / NG;    // n-tiles per wavefront
     static_assert(NT % NG == 0, "n-tiles must split evenly over the wavefronts of an m-tile");
+    static_assert(!KS || (NG == 2 && NKS >= 3), "the k-step split pairs two wavefronts per m-tile");
     constexpr int TB = 32 * NT;              // targets per block
     constexpr int TILE = 2 * NKS * 64;       // bf16x8 vectors of one m-tile of an operator (hi and lo)
-    constexpr int NH = NKS >= 2 ? 2 : 1;      // halves per offset step
-    constexpr int KH0 = (NKS + 1) / 2;        // k-steps of the first half (NH = 1: all)
+    constexpr int KH0 = KS ? 2 * ((NKS + 3) / 4) : (NKS + 1) / 2;  // k-steps of the first half (even with KS)
+    constexpr int NH = NKS > KH0 ? 2 : 1;     // halves per offset step
     constexpr int CPT = (TB * 2 * KH0 + NTH - 1) / NTH;  // 8-coefficient chunks per thread and half
     __shared__ bf16x8 sB[2][KH0][NT][2][64];  // [buffer][k-step in half][n-tile][hi/lo][lane]
     __shared__ int16_t s_dn[189], s_qs[189], s_o[189];
@@ -102,7 +106,8 @@ __global__ __launch_bounds__(64 * NG * mt_of((P + 1) * (P + 1))) void k_m2l_spli
     const int chunk = w >> 3, qt = w & 7;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform, so addresses built from it stay scalar
-    const int mt = wid % MT, n0 = (wid / MT) * NW;
+    const int mt = wid % MT, n0 = KS ? 0 : (wid / MT) * NW;
+    const int kw = KS ? wid / MT : 0;  // KS: this wavefront takes the k-steps ks = kw mod 2
     // Every block walks the 189 offsets e of octant 0 in one fixed order.  For
     // octant qt the geometry is octant 0's mirrored in the axes where qt has a
     // bit set (parent neighbour d -> mirror(d), source octant q -> q ^ qt), and
@@ -234,67 +239,146 @@ __global__ __launch_bounds__(64 * NG * mt_of((P + 1) * (P + 1))) void k_m2l_spli
             l[n] = *reinterpret_cast<const bf16x8 *>(row + (2 * n + 1) * 1024);
         }
     };
-    // Software pipeline, kept in place by scheduling barriers: during k-step q
-    // the A fragments of q + 1 .. q + PF and the B fragments of q + 1 are in
-    // flight.  The
-    // next half is gathered at the first k-step of a half, after that k-step's
-    // A loads, and stored at the fourth (vector loads complete in issue order,
-    // so the A loads issued after the gather already wait for it there).
-    // operator of flat k-step q: step q / NKS, k-step q % NKS
-    auto op_of = [&](int q) -> int { return __builtin_amdgcn_readfirstlane(s_o[q / NKS]); };
-    bf16x8 ah[PF], al[PF];  // A of k-steps q .. q + PF - 1
+    if constexpr (KS) {
+        // Wavefront position j of a step takes k-step ks = 2 j + kw (none when
+        // ks >= NKS); A of positions p + 1 .. p + PF is in flight.  B is read
+        // for the position itself.  Half h covers positions [h JH0, ...), the
+        // same for both wavefronts of an m-tile, so every wavefront meets the
+        // barriers at the same positions.
+        constexpr int NJ = (NKS + 1) / 2, JH0 = KH0 / 2;
+        constexpr int PFJ = 2;
+        constexpr int NSO = (PFJ + NJ - 1) / NJ + 1;  // steps the prefetch of one step reaches
+        // (positions without a k-step, and past the last step, load a valid
+        // fragment that is not used: no branch, so the ring needs no copies)
+        bf16x8 ah[PFJ], al[PFJ];
 #pragma unroll
-    for (int i = 0; i < PF; ++i) {
-        const int o = op_of(i);
-        ah[i] = afrag(o, i % NKS, 0);
-        al[i] = afrag(o, i % NKS, 1);
-    }
-    bf16x8 bh[NW], bl[NW];
-    bfrag(0, 0, bh, bl);
-    for (int st = 0; st < 189; ++st) {
-        asm volatile("" : "+v"(lb));  // recompute the B offsets per step rather than hold them all
+        for (int i = 0; i < PFJ; ++i) {
+            const int ks = min(2 * (i % NJ) + kw, NKS - 1);
+            const int o = __builtin_amdgcn_readfirstlane(s_o[i / NJ]);
+            ah[i] = afrag(o, ks, 0);
+            al[i] = afrag(o, ks, 1);
+        }
+        for (int st = 0; st < 189; ++st) {
+            int so[NSO];  // operators of steps st .. st + NSO - 1, scalar
 #pragma unroll
-        for (int ks = 0; ks < NKS; ++ks) {
-            const int h = ks < KH0 ? 0 : 1;             // half of this k-step
-            const int ksl = ks - h * KH0, kh = h ? NKS - KH0 : KH0;
-            const int bf = NH == 2 ? h : (st & 1);      // its buffer
-            const int nst = (NH == 2 && h == 0) ? st : st + 1, nhf = (NH == 2 && h == 0) ? 1 : 0;  // the next half
-            const int nbf = NH == 2 ? nhf : ((st + 1) & 1);
-            const bool has_next = nst < 189;
-            const int q = st * NKS + ks;
-            const bf16x8 ch = ah[0], cl = al[0];
+            for (int i = 0; i < NSO; ++i) so[i] = __builtin_amdgcn_readfirstlane(st + i < 189 ? s_o[st + i] : 0);
+            asm volatile("" : "+v"(lb));  // recompute the B offsets per step rather than hold them all
 #pragma unroll
-            for (int i = 0; i + 1 < PF; ++i) {
-                ah[i] = ah[i + 1];
-                al[i] = al[i + 1];
+            for (int j = 0; j < NJ; ++j) {
+                const int h = j < JH0 ? 0 : 1, jl = j - h * JH0, jh = h ? NJ - JH0 : JH0;
+                const int ks = 2 * j + kw, ksl = ks - h * KH0;
+                const bool active = ks < NKS;
+                const int bf = NH == 2 ? h : (st & 1);
+                const int nst = (NH == 2 && h == 0) ? st : st + 1, nhf = (NH == 2 && h == 0) ? 1 : 0;
+                const int nbf = NH == 2 ? nhf : ((st + 1) & 1);
+                const bool has_next = nst < 189;
+                const bf16x8 ch = ah[0], cl = al[0];
+#pragma unroll
+                for (int i = 0; i + 1 < PFJ; ++i) {
+                    ah[i] = ah[i + 1];
+                    al[i] = al[i + 1];
+                }
+                {
+                    const int jp = j + PFJ, kp = min(2 * (jp % NJ) + kw, NKS - 1);
+                    ah[PFJ - 1] = afrag(so[jp / NJ], kp, 0);
+                    al[PFJ - 1] = afrag(so[jp / NJ], kp, 1);
+                }
+                if (jl == 0 && has_next) gather(nst, nhf);
+                bf16x8 bh[NW], bl[NW];
+                if (active) bfrag(bf, ksl, bh, bl);
+                __builtin_amdgcn_sched_barrier(0);
+                if (active) {
+#pragma unroll
+                    for (int n = 0; n < NW; ++n) acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ch, bh[n], acc[n], 0, 0, 0);
+#pragma unroll
+                    for (int n = 0; n < NW; ++n) acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ch, bl[n], acc[n], 0, 0, 0);
+#pragma unroll
+                    for (int n = 0; n < NW; ++n) acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(cl, bh[n], acc[n], 0, 0, 0);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+                if (jl + 1 == jh) {
+                    if (has_next) stash(nbf, nhf);
+                    __syncthreads();  // the next half is stored and this one is read
+                }
             }
-            if (q + PF < TOT) {
-                const int o = op_of(q + PF);
-                ah[PF - 1] = afrag(o, (ks + PF) % NKS, 0);
-                al[PF - 1] = afrag(o, (ks + PF) % NKS, 1);
-            }
-            if (ksl == 0 && has_next) gather(nst, nhf);
-            bf16x8 nh[NW], nl[NW];
-            if (ksl + 1 < kh) bfrag(bf, ksl + 1, nh, nl);
-            if (ksl == (kh - 1 < 3 ? kh - 1 : 3) && has_next) stash(nbf, nhf);
-            if (ksl + 1 == kh) {
-                // this half's B is in registers and the next half is stored:
-                // one barrier, then the first fragments of the next half
-                __syncthreads();
-                if (has_next) bfrag(nbf, 0, nh, nl);
-            }
-            __builtin_amdgcn_sched_barrier(0);
+        }
+        // add the odd-k-step sums to the even ones through LDS (B is no longer needed)
+        static_assert(MT * NT * 16 * 64 * sizeof(float) <= sizeof(sB), "partial sums must fit in the B buffers");
+        float *red = reinterpret_cast<float *>(&sB[0][0][0][0][0]);
+        if (kw == 1)
 #pragma unroll
-            for (int n = 0; n < NW; ++n) acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ch, bh[n], acc[n], 0, 0, 0);
+            for (int n = 0; n < NW; ++n)
 #pragma unroll
-            for (int n = 0; n < NW; ++n) acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ch, bl[n], acc[n], 0, 0, 0);
+                for (int r = 0; r < 16; ++r) red[((mt * NT + n) * 16 + r) * 64 + lane] = acc[n][r];
+        __syncthreads();
+        if (kw == 1) return;  // no barrier follows
 #pragma unroll
-            for (int n = 0; n < NW; ++n) acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(cl, bh[n], acc[n], 0, 0, 0);
-            __builtin_amdgcn_sched_barrier(0);
+        for (int n = 0; n < NW; ++n)
 #pragma unroll
-            for (int n = 0; n < NW; ++n) {
-                bh[n] = nh[n];
-                bl[n] = nl[n];
+            for (int r = 0; r < 16; ++r) acc[n][r] += red[((mt * NT + n) * 16 + r) * 64 + lane];
+    } else {
+        // Software pipeline, kept in place by scheduling barriers: during k-step q
+        // the A fragments of q + 1 .. q + PF and the B fragments of q + 1 are in
+        // flight.  The next half is gathered at the first k-step of a half, after
+        // that k-step's A loads, and stored at the fourth (vector loads complete in
+        // issue order, so the A loads issued after the gather already wait for it
+        // there).
+        // operator of flat k-step q: step q / NKS, k-step q % NKS
+        auto op_of = [&](int q) -> int { return __builtin_amdgcn_readfirstlane(s_o[q / NKS]); };
+        bf16x8 ah[PF], al[PF];  // A of k-steps q .. q + PF - 1
+    #pragma unroll
+        for (int i = 0; i < PF; ++i) {
+            const int o = op_of(i);
+            ah[i] = afrag(o, i % NKS, 0);
+            al[i] = afrag(o, i % NKS, 1);
+        }
+        bf16x8 bh[NW], bl[NW];
+        bfrag(0, 0, bh, bl);
+        for (int st = 0; st < 189; ++st) {
+            asm volatile("" : "+v"(lb));  // recompute the B offsets per step rather than hold them all
+    #pragma unroll
+            for (int ks = 0; ks < NKS; ++ks) {
+                const int h = ks < KH0 ? 0 : 1;             // half of this k-step
+                const int ksl = ks - h * KH0, kh = h ? NKS - KH0 : KH0;
+                const int bf = NH == 2 ? h : (st & 1);      // its buffer
+                const int nst = (NH == 2 && h == 0) ? st : st + 1, nhf = (NH == 2 && h == 0) ? 1 : 0;  // the next half
+                const int nbf = NH == 2 ? nhf : ((st + 1) & 1);
+                const bool has_next = nst < 189;
+                const int q = st * NKS + ks;
+                const bf16x8 ch = ah[0], cl = al[0];
+    #pragma unroll
+                for (int i = 0; i + 1 < PF; ++i) {
+                    ah[i] = ah[i + 1];
+                    al[i] = al[i + 1];
+                }
+                if (q + PF < TOT) {
+                    const int o = op_of(q + PF);
+                    ah[PF - 1] = afrag(o, (ks + PF) % NKS, 0);
+                    al[PF - 1] = afrag(o, (ks + PF) % NKS, 1);
+                }
+                if (ksl == 0 && has_next) gather(nst, nhf);
+                bf16x8 nh[NW], nl[NW];
+                if (ksl + 1 < kh) bfrag(bf, ksl + 1, nh, nl);
+                if (ksl == (kh - 1 < 3 ? kh - 1 : 3) && has_next) stash(nbf, nhf);
+                if (ksl + 1 == kh) {
+                    // this half's B is in registers and the next half is stored:
+                    // one barrier, then the first fragments of the next half
+                    __syncthreads();
+                    if (has_next) bfrag(nbf, 0, nh, nl);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+    #pragma unroll
+                for (int n = 0; n < NW; ++n) acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ch, bh[n], acc[n], 0, 0, 0);
+    #pragma unroll
+                for (int n = 0; n < NW; ++n) acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ch, bl[n], acc[n], 0, 0, 0);
+    #pragma unroll
+                for (int n = 0; n < NW; ++n) acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(cl, bh[n], acc[n], 0, 0, 0);
+                __builtin_amdgcn_sched_barrier(0);
+    #pragma unroll
+                for (int n = 0; n < NW; ++n) {
+                    bh[n] = nh[n];
+                    bl[n] = nl[n];
+                }
             }
         }
     }
@@ -331,8 +415,9 @@ template <int P>
 void launch_split(Ctx &c, const int32_t *parents, int np) {
     constexpr int K = (P + 1) * (P + 1);
     constexpr int NT = 4, NG = 2;  // 128 targets, 12 wavefronts: 3 per SIMD
+    constexpr bool KS = ks_of(K) >= 3;
     const unsigned grid = 8 * ceil_div(np, 32 * NT);  // one block per (parent chunk, child octant)
-    k_m2l_split<P, NT, NG><<<grid, 64 * NG * mt_of(K), 0, c.stream>>>(parents, np, c.t.child, c.t.count, c.t.nbr27,
+    k_m2l_split<P, NT, NG, KS><<<grid, 64 * NG * mt_of(K), 0, c.stream>>>(parents, np, c.t.child, c.t.count, c.t.nbr27,
                                                                      reinterpret_cast<const uint4 *>(c.msplit.p), c.L32,
                                                                      reinterpret_cast<const bf16x8 *>(c.opsplit.p), c.ldk);
     VG_LAUNCH_CHECK();
