@@ -194,91 +194,113 @@ __global__ __launch_bounds__(128) void k_ops(const double *__restrict__ harm, in
 }
 
 // ---------------------------------------------------------------------------
-// P2M: one wavefront per leaf; lane i evaluates R_n^m at its particle, the
-// wavefront then sums the 64 rows column by column.
+// P2M: one wavefront per leaf.  Lane = g NM + i, i < NM = p / 2 + 1: particle
+// group g (G = 64 / NM groups, particles g, g + G, ...) and the order pair
+// (m1, m2) = (i, p - i), so that every lane walks the same number of
+// recurrence steps: R_{m1}^{m1} .. R_p^{m1} (p - m1 + 1 values), then
+// R_{m2}^{m2} .. R_p^{m2} (p - m2 + 1; none when m2 = m1), p + 2 in all.
+// Every lane forms the diagonal R_q^q, q <= p, and keeps the two it starts
+// from; step s accumulates m_i conj(R) into accumulator s.  The groups are
+// then summed in fixed order through LDS by all 64 lanes.
 // ---------------------------------------------------------------------------
-// One wavefront per leaf, 4 per block.  Lane = g (p+1) + m: particle group g
-// (G = 64 / (p+1) groups, particles g, g+G, ...) and order m.  A lane forms
-// R_m^m by the diagonal recurrence and R_n^m, n = m..p, by the upward one,
-// and accumulates m_i conj(R_n^m) in registers; the groups are then summed in
-// fixed order through LDS.
 template <class R>
 __global__ __launch_bounds__(256) void k_p2m(const int32_t *__restrict__ leaves, int nleaves, const int8_t *__restrict__ level,
                                              const uint32_t *__restrict__ start, const uint32_t *__restrict__ count, const double *__restrict__ bmin,
                                              const double *__restrict__ bmax, const double *__restrict__ xs64, double wx0, int p, int K, int ldk,
                                              R *__restrict__ M) {
-    constexpr int NP = MAXP + 1;
-    __shared__ R red[4][64][2 * NP];
+    constexpr int NS = MAXP + 2;  // recurrence steps per lane
+    __shared__ R red[4][64][2 * NS];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int li = blockIdx.x * 4 + w;
     if (li >= nleaves) return;  // whole wavefront: no barrier below spans wavefronts
     const int32_t c = leaves[li];
     if (level[c] < 2) return;  // solver_ref.hpp:187-188
-    const int P1 = p + 1, G = 64 / P1;
-    const int m = lane % P1, g = lane / P1;
+    const int NM = p / 2 + 1, G = 64 / NM;
+    const int i = lane % NM, g = lane / NM;
     const bool act = g < G;
+    const int m1 = i, m2 = p - i;
+    const int len1 = p - m1 + 1, len2 = m2 != m1 ? p - m2 + 1 : 0, tot = len1 + len2;
     const uint32_t s = start[c], cnt = count[c];
     const double iw = 1.0 / ldexp(wx0, -level[c]);
     const double cx = (bmin[3 * c] + bmax[3 * c]) / 2, cy = (bmin[3 * c + 1] + bmax[3 * c + 1]) / 2, cz = (bmin[3 * c + 2] + bmax[3 * c + 2]) / 2;
-    R c1[NP], c2[NP], are[NP], aim[NP];
+    // recurrence coefficients of each step (unused at the two chain starts)
+    R c1[NS], c2[NS], are[NS], aim[NS];
 #pragma unroll
-    for (int n = 0; n < NP; ++n) {
-        c1[n] = static_cast<R>(cR1[n * (MAXP + 1) + (act ? m : 0)]);
-        c2[n] = static_cast<R>(cR2[n * (MAXP + 1) + (act ? m : 0)]);
-        are[n] = 0;
-        aim[n] = 0;
+    for (int k = 0; k < NS; ++k) {
+        const int n = k < len1 ? m1 + k : m2 + (k - len1), m = k < len1 ? m1 : m2;
+        const int idx = (k < tot && n <= MAXP) ? n * (MAXP + 1) + m : 0;
+        c1[k] = static_cast<R>(cR1[idx]);
+        c2[k] = static_cast<R>(cR2[idx]);
+        are[k] = 0;
+        aim[k] = 0;
     }
     if (act) {
-        for (uint32_t i = g; i < cnt; i += G) {
-            const uint32_t k = s + i;
+        for (uint32_t j = g; j < cnt; j += G) {
+            const uint32_t k = s + j;
             const R x = static_cast<R>((xs64[4 * k] - cx) * iw), y = static_cast<R>((xs64[4 * k + 1] - cy) * iw),
                     z = static_cast<R>((xs64[4 * k + 2] - cz) * iw);
             const R mi = static_cast<R>(xs64[4 * k + 3]);
             const R r2 = x * x + y * y + z * z;
-            R dre = 1, dim = 0;  // R_m^m
-            for (int q = 1; q <= m; ++q) {
+            // diagonal R_q^q, q = 1..p (the same for every lane), keeping q = m1 and q = m2
+            R dre = 1, dim = 0, d1re = 1, d1im = 0, d2re = 1, d2im = 0;
+            for (int q = 1; q <= p; ++q) {
                 const R d = static_cast<R>(cDiag[q]);
                 const R nre = d * (x * dre - y * dim), nim = d * (x * dim + y * dre);
                 dre = nre;
                 dim = nim;
-            }
-            R bre = 0, bim = 0, cre = dre, cim = dim;  // R_{n-2}^m, R_{n-1}^m
-#pragma unroll
-            for (int n = 0; n < NP; ++n) {
-                if (n < m || n > p) continue;
-                if (n > m) {
-                    const R nre = c1[n] * z * cre - c2[n] * r2 * bre, nim = c1[n] * z * cim - c2[n] * r2 * bim;
-                    bre = cre;
-                    bim = cim;
-                    cre = nre;
-                    cim = nim;
+                if (q == m1) {
+                    d1re = dre;
+                    d1im = dim;
                 }
-                are[n] += mi * cre;
-                aim[n] -= mi * cim;  // conj
+                if (q == m2) {
+                    d2re = dre;
+                    d2im = dim;
+                }
+            }
+            R bre = 0, bim = 0, cre = 0, cim = 0;  // R_{n-2}^m, R_{n-1}^m
+#pragma unroll
+            for (int st = 0; st < NS; ++st) {
+                if (st > p + 1) break;
+                if (st == 0) {
+                    cre = d1re;
+                    cim = d1im;
+                } else {
+                    // the second chain starts at step len1: R_{m2}^{m2}, with R_{m2-1}^{m2} = 0
+                    const bool rs = st == len1;
+                    const R nre = c1[st] * z * cre - c2[st] * r2 * bre, nim = c1[st] * z * cim - c2[st] * r2 * bim;
+                    bre = rs ? R(0) : cre;
+                    bim = rs ? R(0) : cim;
+                    cre = rs ? d2re : nre;
+                    cim = rs ? d2im : nim;
+                }
+                if (st < tot) {
+                    are[st] += mi * cre;
+                    aim[st] -= mi * cim;  // conj
+                }
             }
         }
     }
 #pragma unroll
-    for (int n = 0; n < NP; ++n) {
-        red[w][lane][2 * n] = are[n];
-        red[w][lane][2 * n + 1] = aim[n];
+    for (int k = 0; k < NS; ++k) {
+        red[w][lane][2 * k] = are[k];
+        red[w][lane][2 * k + 1] = aim[k];
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    if (lane < P1) {  // lane m sums the groups of order m
-        R* out = M + static_cast<size_t>(c) * ldk;
-        for (int n = m; n <= p; ++n) {
-            R sre = 0, sim = 0;
-            for (int gg = 0; gg < G; ++gg) {
-                sre += red[w][gg * P1 + m][2 * n];
-                sim += red[w][gg * P1 + m][2 * n + 1];
-            }
-            out[kidx_re(n, m)] = sre;
-            if (m > 0) out[kidx_im(n, m)] = sim;
-        }
+    // output (i, step, part) summed over the groups in fixed order
+    R *out = M + static_cast<size_t>(c) * ldk;
+    for (int o = lane; o < NM * NS * 2; o += 64) {
+        const int oi = o / (2 * NS), ost = (o / 2) % NS, part = o & 1;
+        const int om1 = oi, om2 = p - oi, ol1 = p - om1 + 1, ol2 = om2 != om1 ? p - om2 + 1 : 0;
+        if (ost >= ol1 + ol2) continue;
+        const int n = ost < ol1 ? om1 + ost : om2 + (ost - ol1), m = ost < ol1 ? om1 : om2;
+        if (part == 1 && m == 0) continue;
+        R sum = 0;
+        for (int gg = 0; gg < G; ++gg) sum += red[w][gg * NM + oi][2 * ost + part];
+        out[part == 0 ? kidx_re(n, m) : kidx_im(n, m)] = sum;
     }
-    for (int q = K + lane; q < ldk; q += 64) M[static_cast<size_t>(c) * ldk + q] = R(0);
+    for (int q = K + lane; q < ldk; q += 64) out[q] = R(0);
 }
 
 // M2M for the internal cells of one level (solver_ref.hpp:196-197).
