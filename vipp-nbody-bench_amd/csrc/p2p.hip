@@ -42,10 +42,11 @@ __device__ inline double rsq(double x) { return 1.0 / sqrt(x); }
 
 // Sources [src0, src1) of the leaf's flattened list against targets pt0 ..
 // pt0+nt-1.  out == nullptr: write the final G-scaled result to acc; else
-// write the raw partial sums to out[t].
-template <class R, bool SELF_GUARD, bool POT, int T, class V4>
+// write the raw partial sums to out[t].  UM (every particle has mass mu): the
+// pair terms leave the mass out and the sums are scaled by mu once.
+template <class R, bool SELF_GUARD, bool POT, bool UM, int T, class V4>
 __device__ __forceinline__ void p2p_leaf(V4 *tile, V4 *red, const uint32_t *ent_start, const uint32_t *ent_off, int E, uint32_t src0, uint32_t src1,
-                                         uint32_t nt, uint32_t pt0, const V4 *__restrict__ xs, R eps2, R G, V4 *__restrict__ acc,
+                                         uint32_t nt, uint32_t pt0, const V4 *__restrict__ xs, R eps2, R G, R mu, V4 *__restrict__ acc,
                                          V4 *__restrict__ out) {
     constexpr int CH = sizeof(R) == 4 ? 1024 : 512;
     const int tid = threadIdx.x;
@@ -92,7 +93,7 @@ __device__ __forceinline__ void p2p_leaf(V4 *tile, V4 *red, const uint32_t *ent_
                     const R r2 = eps2 + dx * dx + dy * dy + dz * dz;
                     R rinv = rsq(r2);
                     if (SELF_GUARD) rinv = r2 > R(0) ? rinv : R(0);
-                    const R mr = sj[u].w * rinv;
+                    const R mr = UM ? rinv : sj[u].w * rinv;
                     const R mr3 = mr * rinv * rinv;
                     ax[q] += mr3 * dx;
                     ay[q] += mr3 * dy;
@@ -108,7 +109,7 @@ __device__ __forceinline__ void p2p_leaf(V4 *tile, V4 *red, const uint32_t *ent_
                 const R r2 = eps2 + dx * dx + dy * dy + dz * dz;
                 R rinv = rsq(r2);
                 if (SELF_GUARD) rinv = r2 > R(0) ? rinv : R(0);
-                const R mr = sj.w * rinv;
+                const R mr = UM ? rinv : sj.w * rinv;
                 const R mr3 = mr * rinv * rinv;
                 ax[q] += mr3 * dx;
                 ay[q] += mr3 * dy;
@@ -133,6 +134,12 @@ __device__ __forceinline__ void p2p_leaf(V4 *tile, V4 *red, const uint32_t *ent_
             sz += v.z;
             sp += v.w;
         }
+        if (UM) {
+            sx *= mu;
+            sy *= mu;
+            sz *= mu;
+            sp *= mu;
+        }
         if (out) {
             out[t] = V4{sx, sy, sz, sp};
         } else {
@@ -146,12 +153,12 @@ __device__ __forceinline__ void p2p_leaf(V4 *tile, V4 *red, const uint32_t *ent_
 // group take targets ti, ti+TPW, ... (up to TMAX per lane) and read the same
 // source at each step (an LDS broadcast).  The block's 4 x (64/TPW) groups
 // split every staged tile of sources.
-template <class R, bool SELF_GUARD, bool POT, int T>
+template <class R, bool SELF_GUARD, bool POT, bool UM, int T>
 __global__ __launch_bounds__(P2P_THREADS) void k_p2p(const int2 *__restrict__ items, const int32_t *__restrict__ leaves, const uint32_t *__restrict__ nchunk,
                                                      const uint32_t *__restrict__ item_off, const int8_t *__restrict__ level,
                                                      const uint32_t *__restrict__ start, const uint32_t *__restrict__ count, const int32_t *__restrict__ anc,
                                                      const int32_t *__restrict__ ni, const int32_t *__restrict__ ni_cnt,
-                                                     const typename Vec4<R>::T *__restrict__ xs, R eps2, R G, typename Vec4<R>::T *__restrict__ acc,
+                                                     const typename Vec4<R>::T *__restrict__ xs, R eps2, R G, R mu, typename Vec4<R>::T *__restrict__ acc,
                                                      typename Vec4<R>::T *__restrict__ partial, uint32_t pstride) {
     using V4 = typename Vec4<R>::T;
     constexpr int CH = sizeof(R) == 4 ? 1024 : 512;
@@ -208,11 +215,11 @@ __global__ __launch_bounds__(P2P_THREADS) void k_p2p(const int2 *__restrict__ it
     const uint32_t src0 = static_cast<uint32_t>(item.y) * P2P_CHUNK, src1 = min(total, src0 + P2P_CHUNK);
     V4 *out = nchunk[item.x] > 1 ? partial + static_cast<size_t>(item_off[item.x] + item.y) * pstride : nullptr;
     if (T < 4 || nt <= 256) {
-        p2p_leaf<R, SELF_GUARD, POT, T>(tile, red, ent_start, ent_off, E, src0, src1, nt, pt0, xs, eps2, G, acc, out);
+        p2p_leaf<R, SELF_GUARD, POT, UM, T>(tile, red, ent_start, ent_off, E, src0, src1, nt, pt0, xs, eps2, G, mu, acc, out);
     } else {  // leaves above 256 points: batches of 256 targets, each streaming the sources again
         for (uint32_t tb = 0; tb < nt; tb += 256) {
-            p2p_leaf<R, SELF_GUARD, POT, 4>(tile, red, ent_start, ent_off, E, src0, src1, min(nt - tb, 256u), pt0 + tb, xs, eps2, G, acc,
-                                            out ? out + tb : nullptr);
+            p2p_leaf<R, SELF_GUARD, POT, UM, 4>(tile, red, ent_start, ent_off, E, src0, src1, min(nt - tb, 256u), pt0 + tb, xs, eps2, G, mu, acc,
+                                                out ? out + tb : nullptr);
             __syncthreads();
         }
     }
@@ -364,6 +371,10 @@ void p2p_t(Ctx &c, bool potential) {
     }
     const R eps2 = static_cast<R>(c.eps * c.eps);
     const bool guard = !(eps2 > 0);
+    // one mass for every particle (no per-particle masses uploaded): force
+    // kernels take it out of the pair terms (the potential keeps the general path)
+    const bool um = !c.has_masses;
+    const R mu = static_cast<R>(c.mass);
     const int nl = static_cast<int>(t.nleaves);
     const int nb = static_cast<int>(ceil_div(nl, 256));
     c.p2p_scal.ensure(16);
@@ -402,30 +413,34 @@ void p2p_t(Ctx &c, bool potential) {
         if (!hs[cls]) continue;
         const dim3 grid(hs[cls]);
         const int2 *it = items + hbase[cls];
-#define VG_P2P_T(G_, P_, T_)                                                                                                                         \
-    k_p2p<R, G_, P_, T_><<<grid, P2P_THREADS, 0, c.stream>>>(it, t.leaf_ids, c.p2p_nchunk, c.p2p_item_off, t.level, t.start, t.count, t.anc, t.ni, \
-                                                             t.ni_cnt, xs, eps2, static_cast<R>(c.G), acc, partial, pstride)
-#define VG_P2P(G_, P_)           \
-    do {                         \
-        if (cls == 0)            \
-            VG_P2P_T(G_, P_, 1); \
-        else if (cls == 1)       \
-            VG_P2P_T(G_, P_, 2); \
-        else if (cls == 2)       \
-            VG_P2P_T(G_, P_, 3); \
-        else                     \
-            VG_P2P_T(G_, P_, 4); \
+#define VG_P2P_T(G_, P_, U_, T_)                                                                                                                \
+    k_p2p<R, G_, P_, U_, T_><<<grid, P2P_THREADS, 0, c.stream>>>(it, t.leaf_ids, c.p2p_nchunk, c.p2p_item_off, t.level, t.start, t.count, t.anc, \
+                                                                 t.ni, t.ni_cnt, xs, eps2, static_cast<R>(c.G), mu, acc, partial, pstride)
+#define VG_P2P(G_, P_, U_)           \
+    do {                             \
+        if (cls == 0)                \
+            VG_P2P_T(G_, P_, U_, 1); \
+        else if (cls == 1)           \
+            VG_P2P_T(G_, P_, U_, 2); \
+        else if (cls == 2)           \
+            VG_P2P_T(G_, P_, U_, 3); \
+        else                         \
+            VG_P2P_T(G_, P_, U_, 4); \
     } while (0)
         if (guard) {
             if (potential)
-                VG_P2P(true, true);
+                VG_P2P(true, true, false);
+            else if (um)
+                VG_P2P(true, false, true);
             else
-                VG_P2P(true, false);
+                VG_P2P(true, false, false);
         } else {
             if (potential)
-                VG_P2P(false, true);
+                VG_P2P(false, true, false);
+            else if (um)
+                VG_P2P(false, false, true);
             else
-                VG_P2P(false, false);
+                VG_P2P(false, false, false);
         }
 #undef VG_P2P
 #undef VG_P2P_T
