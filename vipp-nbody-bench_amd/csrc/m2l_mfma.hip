@@ -266,7 +266,11 @@ __global__ __launch_bounds__(64 * NG * mt_of((P + 1) * (P + 1))) void k_m2l_spli
 #pragma unroll
             for (int j = 0; j < NJ; ++j) {
                 const int h = j < JH0 ? 0 : 1, jl = j - h * JH0, jh = h ? NJ - JH0 : JH0;
-                const int ks = 2 * j + kw, ksl = ks - h * KH0;
+                // with NKS odd the last k-step is shared: the even wavefront forms
+                // hi*hi + hi*lo, the odd one lo*hi, so neither waits for the other
+                constexpr bool SHARED_LAST = (NKS & 1) != 0;
+                const bool last = SHARED_LAST && j == NJ - 1;
+                const int ks = last ? NKS - 1 : 2 * j + kw, ksl = ks - h * KH0;
                 const bool active = ks < NKS;
                 const int bf = NH == 2 ? h : (st & 1);
                 const int nst = (NH == 2 && h == 0) ? st : st + 1, nhf = (NH == 2 && h == 0) ? 1 : 0;
@@ -287,11 +291,13 @@ __global__ __launch_bounds__(64 * NG * mt_of((P + 1) * (P + 1))) void k_m2l_spli
                 bf16x8 bh[NW], bl[NW];
                 if (active) bfrag(bf, ksl, bh, bl);
                 __builtin_amdgcn_sched_barrier(0);
-                if (active) {
+                if (!last || kw == 0) {
 #pragma unroll
                     for (int n = 0; n < NW; ++n) acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ch, bh[n], acc[n], 0, 0, 0);
 #pragma unroll
                     for (int n = 0; n < NW; ++n) acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ch, bl[n], acc[n], 0, 0, 0);
+                }
+                if (!last || kw == 1) {
 #pragma unroll
                     for (int n = 0; n < NW; ++n) acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(cl, bh[n], acc[n], 0, 0, 0);
                 }
