@@ -38,6 +38,7 @@ void swap_expansions(Ctx &c, ExpSet &e) {
     c.M32.swap(e.M32);
     c.L32.swap(e.L32);
     c.opsplit.swap(e.opsplit);
+    c.opsplit_ml.swap(e.opsplit_ml);
     c.msplit.swap(e.msplit);
 }
 void fmm_constants_init();

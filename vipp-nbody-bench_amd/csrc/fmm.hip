@@ -654,6 +654,10 @@ inline int pick_pb(int64_t nparents, int maxpb) {
 template <class R>
 void m2m_level(Ctx &c, const int32_t *ids, int64_t n) {
     const TreeDev &t = c.t;
+    if (sizeof(R) == 4 && c.prec == 2) {  // split mode: on the matrix cores
+        ml_split_level(c, ids, n, true);
+        return;
+    }
     const int pb = pick_pb(n, 8);  // 8 children x PB staged expansions in LDS
     if (n < 1024) {  // few parents: one wide block each
         k_m2m_wide<R><<<static_cast<unsigned>(n), 1024, 0, c.stream>>>(ids, t.child, t.count, Bufs<R>::op(c), c.K, c.ldk, Bufs<R>::M(c));
@@ -671,6 +675,10 @@ void m2m_level(Ctx &c, const int32_t *ids, int64_t n) {
 template <class R>
 void l2l_level(Ctx &c, const int32_t *ids, int64_t n) {
     const TreeDev &t = c.t;
+    if (sizeof(R) == 4 && c.prec == 2) {
+        ml_split_level(c, ids, n, false);
+        return;
+    }
     const int pb = pick_pb(n, 16);
     if (n < 1024) {
         k_l2l_wide<R><<<static_cast<unsigned>(n), 1024, 0, c.stream>>>(ids, t.child, t.count, Bufs<R>::op(c), c.K, c.ldk, Bufs<R>::L(c));

@@ -50,7 +50,7 @@ struct ExpSet {
     int p = 0, K = 0, ldk = 0;
     DevBuf<double> harm, op64;
     DevBuf<float> op32, M32, L32;
-    DevBuf<uint16_t> opsplit, msplit;
+    DevBuf<uint16_t> opsplit, opsplit_ml, msplit;
 };
 
 // learned closure state (closure.hip)
@@ -129,6 +129,8 @@ struct Ctx {
     DevBuf<double> op64;        // M2M (8), L2L (8), M2L (343) blocks of K*K
     DevBuf<float> op32;
     DevBuf<uint16_t> opsplit;   // M2L operators as BF16 hi/lo MFMA fragments
+    DevBuf<uint16_t> opsplit_ml;  // the 8 M2M and 8 L2L operators, the same way
+    DevBuf<float> ml_part;      // split M2M: the 8 octant products of one level before their sum
     DevBuf<uint16_t> msplit;    // multipoles as BF16 hi / lo halves (per cell ldk hi, then ldk lo)
     DevBuf<double> consts;      // A_n^m and recurrence constants (device copy)
     DevBuf<double> opvec;       // operator offset vectors (device copy)
@@ -173,6 +175,7 @@ void closure_fallback_l2p(Ctx &c);            // order-p_hi L2P of the flagged l
 void swap_expansions(Ctx &c, ExpSet &e);      // exchange the context's expansion set with e
 void m2l_list(Ctx &c, const int32_t *parents, int np);      // M2L of the children of the given internal cells
 void l2l_list(Ctx &c, const int32_t *parents, int64_t np);  // L2L into the children of the given internal cells (FP32)
+void ml_split_level(Ctx &c, const int32_t *ids, int64_t n, bool m2m);  // M2M (m2m) or L2L of one level on the matrix cores
 void l2p_list(Ctx &c, const int32_t *leaves, int64_t n, const uint8_t *skip);  // L2P of the given leaves (FP32)
 void direct_subset(Ctx &c, const int64_t *idx, int64_t nidx, double *a, double *pot);  // FP64 O(N nidx) oracle forces
 

@@ -417,6 +417,180 @@ __global__ __launch_bounds__(256) void k_split_m(const float *__restrict__ M, in
     Ms[c * 2 * ldk + ldk + k] = __builtin_bit_cast(uint16_t, l);
 }
 
+// ---------------------------------------------------------------------------
+// M2M and L2L of one tree level on the matrix cores, same split as M2L.  The
+// 8 M2M (child - parent) and 8 L2L (parent - child) operators are stored
+// column-major for the FP32 kernels; their fragments are cut here in the A
+// order of the M2L blocks.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_split_ops_ml(const double *__restrict__ op64, int K, __bf16 *__restrict__ frag, int MT, int NKS) {
+    const int64_t i = blockIdx.x * 256ll + threadIdx.x;
+    const int64_t per = static_cast<int64_t>(MT) * NKS * 64;
+    if (i >= 16 * per) return;
+    const int blk = static_cast<int>(i / per);
+    const int rem = static_cast<int>(i % per);
+    const int mt = rem / (NKS * 64), ks = (rem / 64) % NKS, l = rem % 64;
+    const double *src = op64 + static_cast<int64_t>(blk) * K * K;
+    const int row = mt * 32 + (l & 31);
+    const int64_t vec = (static_cast<int64_t>(blk) * MT + mt) * 2 * NKS * 64 + ks * 64 + l;
+    __bf16 *hi = frag + vec * 8;
+    __bf16 *lo = frag + (vec + NKS * 64) * 8;
+    for (int e = 0; e < 8; ++e) {
+        const int col = ks * 16 + (l >> 5) * 8 + e;
+        const double x = (row < K && col < K) ? src[static_cast<int64_t>(col) * K + row] : 0.0;  // column-major
+        const __bf16 h = static_cast<__bf16>(static_cast<float>(x));
+        hi[e] = h;
+        lo[e] = static_cast<__bf16>(static_cast<float>(x - static_cast<double>(static_cast<float>(h))));
+    }
+}
+
+// Expansions of 32 cells (row stride ldk = 16 NKS; cells[t] < 0: zeros) ->
+// LDS as hi/lo B fragments of one n-tile, the M2L kernel's slot order.  Rows
+// K .. ldk-1 are read as zero whatever the buffer holds there (the locals'
+// padding is never written).
+template <int K, int NKS, int NTH>
+__device__ __forceinline__ void stage_split(bf16x8 (*sB)[2][64], const float *__restrict__ src, int ldk, const int32_t *cells) {
+    for (int c = threadIdx.x; c < 32 * 2 * NKS; c += NTH) {
+        const int t = c / (2 * NKS), k8 = c % (2 * NKS);
+        const int32_t cell = cells[t];
+        float4 a = make_float4(0, 0, 0, 0), b = a;
+        if (cell >= 0) {
+            const float4 *q = reinterpret_cast<const float4 *>(src + static_cast<int64_t>(cell) * ldk + 8 * k8);
+            a = q[0];
+            b = q[1];
+        }
+        float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+        bf16x8 h, l;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            if (8 * k8 + e >= K) v[e] = 0.0f;
+            h[e] = static_cast<__bf16>(v[e]);
+            l[e] = static_cast<__bf16>(v[e] - static_cast<float>(h[e]));
+        }
+        const int ln = ((k8 & 1) * 32 + t) ^ (k8 & 15);
+        sB[k8 >> 1][0][ln] = h;
+        sB[k8 >> 1][1][ln] = l;
+    }
+}
+
+// acc += T (m-tile mt of fragment block fb) x B (LDS), hi*hi + hi*lo + lo*hi
+template <int NKS>
+__device__ __forceinline__ void mfma_split_tile(const bf16x8 *__restrict__ fb, int mt, int lane, bf16x8 (*sB)[2][64], f32x16 &acc) {
+#pragma unroll
+    for (int ks = 0; ks < NKS; ++ks) {
+        const bf16x8 ah = fb[(mt * 2) * NKS * 64 + ks * 64 + lane], al = fb[(mt * 2 + 1) * NKS * 64 + ks * 64 + lane];
+        const int slot = lane ^ ((2 * ks + (lane >> 5)) & 15);
+        const bf16x8 bh = sB[ks][0][slot], bl = sB[ks][1][slot];
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc, 0, 0, 0);
+    }
+}
+
+// C of the wavefront's m-tile -> sO[t][row] (row stride OST, odd, so the 32
+// targets of one row fall in different banks)
+template <int OST>
+__device__ __forceinline__ void park_tile(float *sO, int mt, int lane, const f32x16 &acc) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) sO[(lane & 31) * OST + mt * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)] = acc[r];
+}
+
+// M2M (solver_ref.hpp:196-197), first pass: block = (32 parents, child
+// octant q) forms T_q M(child q) (zero for an empty child) into
+// part[q][parent][row], rows K .. ldk-1 zero (zero rows of the operators).
+template <int P>
+__global__ __launch_bounds__(64 * mt_of((P + 1) * (P + 1))) void k_m2m_split(const int32_t *__restrict__ ids, int n, const int32_t *__restrict__ child,
+                                                                          const uint32_t *__restrict__ count, const bf16x8 *__restrict__ frag,
+                                                                          const float *__restrict__ M, int ldk, float *__restrict__ part) {
+    constexpr int K = (P + 1) * (P + 1), MT = mt_of(K), NKS = ks_of(K), NTH = 64 * MT;
+    constexpr int OST = MT * 32 + 1;
+    __shared__ bf16x8 sB[NKS][2][64];
+    __shared__ float sO[32 * OST];
+    __shared__ int32_t s_cell[32];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int mt = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int q = blockIdx.x & 7, chunk = blockIdx.x >> 3;
+    if (tid < 32) {
+        const int32_t P0 = chunk * 32 + tid < n ? ids[chunk * 32 + tid] : -1;
+        const int32_t cq = P0 >= 0 ? child[P0] + q : -1;
+        s_cell[tid] = (cq >= 0 && count[cq] > 0) ? cq : -1;
+    }
+    __syncthreads();
+    stage_split<K, NKS, NTH>(sB, M, ldk, s_cell);
+    __syncthreads();
+    f32x16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+    mfma_split_tile<NKS>(frag + static_cast<int64_t>(q) * MT * 2 * NKS * 64, mt, lane, sB, acc);
+    park_tile<OST>(sO, mt, lane, acc);
+    __syncthreads();
+    const int nt = min(32, n - chunk * 32);
+    for (int i = tid; i < nt * ldk; i += NTH) {
+        const int t = i / ldk, row = i % ldk;
+        part[(static_cast<int64_t>(q) * n + chunk * 32 + t) * ldk + row] = sO[t * OST + row];
+    }
+}
+
+// M2M, second pass: M(parent) = sum over q = 0..7 of part[q][parent], in that order
+__global__ __launch_bounds__(256) void k_m2m_sum(const int32_t *__restrict__ ids, int64_t n, int ldk, const float *__restrict__ part,
+                                                 float *__restrict__ M) {
+    const int64_t i = blockIdx.x * 256ll + threadIdx.x;
+    if (i >= n * ldk) return;
+    float s = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) s += part[q * n * ldk + i];
+    M[static_cast<int64_t>(ids[i / ldk]) * ldk + i % ldk] = s;
+}
+
+// L2L (solver_ref.hpp:217-218): block = (32 parents, child octant q);
+// L(child q) += T'_q L(parent) for every non-empty child.
+template <int P>
+__global__ __launch_bounds__(64 * mt_of((P + 1) * (P + 1))) void k_l2l_split(const int32_t *__restrict__ ids, int n, const int32_t *__restrict__ child,
+                                                                          const uint32_t *__restrict__ count, const bf16x8 *__restrict__ frag,
+                                                                          float *__restrict__ L, int ldk) {
+    constexpr int K = (P + 1) * (P + 1), MT = mt_of(K), NKS = ks_of(K), NTH = 64 * MT;
+    constexpr int OST = MT * 32 + 1;
+    __shared__ bf16x8 sB[NKS][2][64];
+    __shared__ float sO[32 * OST];
+    __shared__ int32_t s_par[32], s_out[32];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int mt = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int q = blockIdx.x & 7, chunk = blockIdx.x >> 3;
+    if (tid < 32) {
+        const int32_t P0 = chunk * 32 + tid < n ? ids[chunk * 32 + tid] : -1;
+        const int32_t cq = P0 >= 0 ? child[P0] + q : -1;
+        s_par[tid] = P0;
+        s_out[tid] = (cq >= 0 && count[cq] > 0) ? cq : -1;
+    }
+    __syncthreads();
+    stage_split<K, NKS, NTH>(sB, L, ldk, s_par);
+    __syncthreads();
+    f32x16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+    mfma_split_tile<NKS>(frag + static_cast<int64_t>(8 + q) * MT * 2 * NKS * 64, mt, lane, sB, acc);
+    park_tile<OST>(sO, mt, lane, acc);
+    __syncthreads();
+    for (int i = tid; i < 32 * K; i += NTH) {  // each child's row updated contiguously
+        const int t = i / K, row = i % K;
+        if (s_out[t] >= 0) L[static_cast<int64_t>(s_out[t]) * ldk + row] += sO[t * OST + row];
+    }
+}
+
+template <int P>
+void launch_ml(Ctx &c, const int32_t *ids, int64_t n, bool m2m) {
+    constexpr int K = (P + 1) * (P + 1);
+    const unsigned grid = static_cast<unsigned>(ceil_div(n, 32));
+    const bf16x8 *frag = reinterpret_cast<const bf16x8 *>(c.opsplit_ml.p);
+    if (m2m) {
+        c.ml_part.ensure(static_cast<size_t>(8 * n * c.ldk));
+        k_m2m_split<P><<<8 * grid, 64 * mt_of(K), 0, c.stream>>>(ids, static_cast<int>(n), c.t.child, c.t.count, frag, c.M32, c.ldk, c.ml_part);
+        k_m2m_sum<<<ceil_div(n * c.ldk, 256), 256, 0, c.stream>>>(ids, n, c.ldk, c.ml_part, c.M32);
+    } else
+        k_l2l_split<P><<<8 * grid, 64 * mt_of(K), 0, c.stream>>>(ids, static_cast<int>(n), c.t.child, c.t.count, frag, c.L32, c.ldk);
+    VG_LAUNCH_CHECK();
+}
+
 template <int P>
 void launch_split(Ctx &c, const int32_t *parents, int np) {
     constexpr int K = (P + 1) * (P + 1);
@@ -437,7 +611,28 @@ void m2l_split_operators(Ctx &c, int nvec) {
     c.opsplit.ensure(static_cast<size_t>(343 * 2 * per * 8));
     const int64_t n = std::max<int64_t>(static_cast<int64_t>(nvec) * c.K * c.K, 343 * per);
     k_split_ops<<<ceil_div(n, 256), 256, 0, c.stream>>>(c.op64, nvec, c.K, c.op32, reinterpret_cast<__bf16 *>(c.opsplit.p), MT, NKS);
+    c.opsplit_ml.ensure(static_cast<size_t>(16 * 2 * per * 8));
+    k_split_ops_ml<<<ceil_div(16 * per, 256), 256, 0, c.stream>>>(c.op64, c.K, reinterpret_cast<__bf16 *>(c.opsplit_ml.p), MT, NKS);
     VG_LAUNCH_CHECK();
+}
+
+void ml_split_level(Ctx &c, const int32_t *ids, int64_t n, bool m2m) {
+    if (n <= 0) return;
+    switch (c.p) {
+        case 1: launch_ml<1>(c, ids, n, m2m); break;
+        case 2: launch_ml<2>(c, ids, n, m2m); break;
+        case 3: launch_ml<3>(c, ids, n, m2m); break;
+        case 4: launch_ml<4>(c, ids, n, m2m); break;
+        case 5: launch_ml<5>(c, ids, n, m2m); break;
+        case 6: launch_ml<6>(c, ids, n, m2m); break;
+        case 7: launch_ml<7>(c, ids, n, m2m); break;
+        case 8: launch_ml<8>(c, ids, n, m2m); break;
+        case 9: launch_ml<9>(c, ids, n, m2m); break;
+        case 10: launch_ml<10>(c, ids, n, m2m); break;
+        case 11: launch_ml<11>(c, ids, n, m2m); break;
+        case 12: launch_ml<12>(c, ids, n, m2m); break;
+        default: throw Error(1, "split M2M / L2L supports 1 <= p <= 12");
+    }
 }
 
 void fmm_m2l_split_list(Ctx &c, const int32_t *parents, int np) {
