@@ -418,11 +418,21 @@ __global__ __launch_bounds__(256) void k_split_m(const float *__restrict__ M, in
 }
 
 // ---------------------------------------------------------------------------
-// M2M and L2L of one tree level on the matrix cores, same split as M2L.  The
-// 8 M2M (child - parent) and 8 L2L (parent - child) operators are stored
+// M2M and L2L of one tree level on the matrix cores.  Their work is small
+// (8 operators per level instead of 189 offsets), so they take a three-way
+// split, x = h + m + l (BF16 each, 24 significant bits together), and six
+// products, hh + hm + mh + hl + mm + lh, which keeps them at FP32 accuracy.
+// The 8 M2M (child - parent) and 8 L2L (parent - child) operators are stored
 // column-major for the FP32 kernels; their fragments are cut here in the A
-// order of the M2L blocks.
+// order of the M2L blocks, frag[blk][mt][part][ks][lane][8].
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ void split3(float x, __bf16 &h, __bf16 &m, __bf16 &l) {
+    h = static_cast<__bf16>(x);
+    const float r = x - static_cast<float>(h);
+    m = static_cast<__bf16>(r);
+    l = static_cast<__bf16>(r - static_cast<float>(m));
+}
+
 __global__ __launch_bounds__(256) void k_split_ops_ml(const double *__restrict__ op64, int K, __bf16 *__restrict__ frag, int MT, int NKS) {
     const int64_t i = blockIdx.x * 256ll + threadIdx.x;
     const int64_t per = static_cast<int64_t>(MT) * NKS * 64;
@@ -432,24 +442,26 @@ __global__ __launch_bounds__(256) void k_split_ops_ml(const double *__restrict__
     const int mt = rem / (NKS * 64), ks = (rem / 64) % NKS, l = rem % 64;
     const double *src = op64 + static_cast<int64_t>(blk) * K * K;
     const int row = mt * 32 + (l & 31);
-    const int64_t vec = (static_cast<int64_t>(blk) * MT + mt) * 2 * NKS * 64 + ks * 64 + l;
-    __bf16 *hi = frag + vec * 8;
-    __bf16 *lo = frag + (vec + NKS * 64) * 8;
+    const int64_t vec = (static_cast<int64_t>(blk) * MT + mt) * 3 * NKS * 64 + ks * 64 + l;
     for (int e = 0; e < 8; ++e) {
         const int col = ks * 16 + (l >> 5) * 8 + e;
         const double x = (row < K && col < K) ? src[static_cast<int64_t>(col) * K + row] : 0.0;  // column-major
+        // the first part from the FP64 value, the other two from the residual
         const __bf16 h = static_cast<__bf16>(static_cast<float>(x));
-        hi[e] = h;
-        lo[e] = static_cast<__bf16>(static_cast<float>(x - static_cast<double>(static_cast<float>(h))));
+        __bf16 m, lo, rest;
+        split3(static_cast<float>(x - static_cast<double>(static_cast<float>(h))), m, lo, rest);
+        frag[vec * 8 + e] = h;
+        frag[(vec + NKS * 64) * 8 + e] = m;
+        frag[(vec + 2 * NKS * 64) * 8 + e] = lo;
     }
 }
 
 // Expansions of 32 cells (row stride ldk = 16 NKS; cells[t] < 0: zeros) ->
-// LDS as hi/lo B fragments of one n-tile, the M2L kernel's slot order.  Rows
+// LDS as h/m/l B fragments of one n-tile, the M2L kernel's slot order.  Rows
 // K .. ldk-1 are read as zero whatever the buffer holds there (the locals'
 // padding is never written).
 template <int K, int NKS, int NTH>
-__device__ __forceinline__ void stage_split(bf16x8 (*sB)[2][64], const float *__restrict__ src, int ldk, const int32_t *cells) {
+__device__ __forceinline__ void stage_split(bf16x8 (*sB)[3][64], const float *__restrict__ src, int ldk, const int32_t *cells) {
     for (int c = threadIdx.x; c < 32 * 2 * NKS; c += NTH) {
         const int t = c / (2 * NKS), k8 = c % (2 * NKS);
         const int32_t cell = cells[t];
@@ -460,30 +472,39 @@ __device__ __forceinline__ void stage_split(bf16x8 (*sB)[2][64], const float *__
             b = q[1];
         }
         float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-        bf16x8 h, l;
+        bf16x8 h, m, l;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             if (8 * k8 + e >= K) v[e] = 0.0f;
-            h[e] = static_cast<__bf16>(v[e]);
-            l[e] = static_cast<__bf16>(v[e] - static_cast<float>(h[e]));
+            __bf16 he, me, le;
+            split3(v[e], he, me, le);
+            h[e] = he;
+            m[e] = me;
+            l[e] = le;
         }
         const int ln = ((k8 & 1) * 32 + t) ^ (k8 & 15);
         sB[k8 >> 1][0][ln] = h;
-        sB[k8 >> 1][1][ln] = l;
+        sB[k8 >> 1][1][ln] = m;
+        sB[k8 >> 1][2][ln] = l;
     }
 }
 
-// acc += T (m-tile mt of fragment block fb) x B (LDS), hi*hi + hi*lo + lo*hi
+// acc += T (m-tile mt of fragment block fb) x B (LDS): the six products of
+// the three-way split down to 2^-16 relative, smallest first
 template <int NKS>
-__device__ __forceinline__ void mfma_split_tile(const bf16x8 *__restrict__ fb, int mt, int lane, bf16x8 (*sB)[2][64], f32x16 &acc) {
+__device__ __forceinline__ void mfma_split_tile(const bf16x8 *__restrict__ fb, int mt, int lane, bf16x8 (*sB)[3][64], f32x16 &acc) {
 #pragma unroll
     for (int ks = 0; ks < NKS; ++ks) {
-        const bf16x8 ah = fb[(mt * 2) * NKS * 64 + ks * 64 + lane], al = fb[(mt * 2 + 1) * NKS * 64 + ks * 64 + lane];
+        const bf16x8 ah = fb[(mt * 3) * NKS * 64 + ks * 64 + lane], am = fb[(mt * 3 + 1) * NKS * 64 + ks * 64 + lane],
+                     al = fb[(mt * 3 + 2) * NKS * 64 + ks * 64 + lane];
         const int slot = lane ^ ((2 * ks + (lane >> 5)) & 15);
-        const bf16x8 bh = sB[ks][0][slot], bl = sB[ks][1][slot];
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
+        const bf16x8 bh = sB[ks][0][slot], bm = sB[ks][1][slot], bl = sB[ks][2][slot];
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(am, bm, acc, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bm, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(am, bh, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
     }
 }
 
@@ -504,7 +525,7 @@ __global__ __launch_bounds__(64 * mt_of((P + 1) * (P + 1))) void k_m2m_split(con
                                                                           const float *__restrict__ M, int ldk, float *__restrict__ part) {
     constexpr int K = (P + 1) * (P + 1), MT = mt_of(K), NKS = ks_of(K), NTH = 64 * MT;
     constexpr int OST = MT * 32 + 1;
-    __shared__ bf16x8 sB[NKS][2][64];
+    __shared__ bf16x8 sB[NKS][3][64];
     __shared__ float sO[32 * OST];
     __shared__ int32_t s_cell[32];
     const int tid = threadIdx.x, lane = tid & 63;
@@ -521,7 +542,7 @@ __global__ __launch_bounds__(64 * mt_of((P + 1) * (P + 1))) void k_m2m_split(con
     f32x16 acc;
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
-    mfma_split_tile<NKS>(frag + static_cast<int64_t>(q) * MT * 2 * NKS * 64, mt, lane, sB, acc);
+    mfma_split_tile<NKS>(frag + static_cast<int64_t>(q) * MT * 3 * NKS * 64, mt, lane, sB, acc);
     park_tile<OST>(sO, mt, lane, acc);
     __syncthreads();
     const int nt = min(32, n - chunk * 32);
@@ -550,7 +571,7 @@ __global__ __launch_bounds__(64 * mt_of((P + 1) * (P + 1))) void k_l2l_split(con
                                                                           float *__restrict__ L, int ldk) {
     constexpr int K = (P + 1) * (P + 1), MT = mt_of(K), NKS = ks_of(K), NTH = 64 * MT;
     constexpr int OST = MT * 32 + 1;
-    __shared__ bf16x8 sB[NKS][2][64];
+    __shared__ bf16x8 sB[NKS][3][64];
     __shared__ float sO[32 * OST];
     __shared__ int32_t s_par[32], s_out[32];
     const int tid = threadIdx.x, lane = tid & 63;
@@ -568,7 +589,7 @@ __global__ __launch_bounds__(64 * mt_of((P + 1) * (P + 1))) void k_l2l_split(con
     f32x16 acc;
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
-    mfma_split_tile<NKS>(frag + static_cast<int64_t>(8 + q) * MT * 2 * NKS * 64, mt, lane, sB, acc);
+    mfma_split_tile<NKS>(frag + static_cast<int64_t>(8 + q) * MT * 3 * NKS * 64, mt, lane, sB, acc);
     park_tile<OST>(sO, mt, lane, acc);
     __syncthreads();
     for (int i = tid; i < 32 * K; i += NTH) {  // each child's row updated contiguously
@@ -611,7 +632,7 @@ void m2l_split_operators(Ctx &c, int nvec) {
     c.opsplit.ensure(static_cast<size_t>(343 * 2 * per * 8));
     const int64_t n = std::max<int64_t>(static_cast<int64_t>(nvec) * c.K * c.K, 343 * per);
     k_split_ops<<<ceil_div(n, 256), 256, 0, c.stream>>>(c.op64, nvec, c.K, c.op32, reinterpret_cast<__bf16 *>(c.opsplit.p), MT, NKS);
-    c.opsplit_ml.ensure(static_cast<size_t>(16 * 2 * per * 8));
+    c.opsplit_ml.ensure(static_cast<size_t>(16 * 3 * per * 8));
     k_split_ops_ml<<<ceil_div(16 * per, 256), 256, 0, c.stream>>>(c.op64, c.K, reinterpret_cast<__bf16 *>(c.opsplit_ml.p), MT, NKS);
     VG_LAUNCH_CHECK();
 }
