@@ -32,14 +32,14 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 __host__ __device__ constexpr int mt_of(int K) { return (K + 31) / 32; }
 __host__ __device__ constexpr int ks_of(int K) { return (K + 15) / 16; }
 
-// FP64 operators (row-major K x K per offset) -> FP32 copies (all blocks) and
-// BF16 hi/lo A fragments of the M2L blocks: frag[o][mt][h][ks][lane][e], so
+// FP64 operators (row-major K x K per offset) -> BF16 hi/lo A fragments of
+// the M2L blocks: frag[o][mt][h][ks][lane][e], so
 // that any run of consecutive m-tiles (hi and lo) is one contiguous range.
 __global__ __launch_bounds__(256) void k_split_ops(const double *__restrict__ op64, int nblk, int K, float *__restrict__ op32, __bf16 *__restrict__ frag,
                                                    int MT, int NKS) {
     const int64_t i = blockIdx.x * 256ll + threadIdx.x;
     const int64_t kk = static_cast<int64_t>(K) * K;
-    if (i < nblk * kk) op32[i] = static_cast<float>(op64[i]);
+    if (op32 && i < nblk * kk) op32[i] = static_cast<float>(op64[i]);  // FP32 copies only when asked for
     const int64_t per = static_cast<int64_t>(MT) * NKS * 64;  // lanes per operator
     if (i >= 343 * per) return;
     const int o = static_cast<int>(i / per);
@@ -630,8 +630,9 @@ void m2l_split_operators(Ctx &c, int nvec) {
     const int MT = mt_of(c.K), NKS = ks_of(c.K);
     const int64_t per = static_cast<int64_t>(MT) * NKS * 64;
     c.opsplit.ensure(static_cast<size_t>(343 * 2 * per * 8));
-    const int64_t n = std::max<int64_t>(static_cast<int64_t>(nvec) * c.K * c.K, 343 * per);
-    k_split_ops<<<ceil_div(n, 256), 256, 0, c.stream>>>(c.op64, nvec, c.K, c.op32, reinterpret_cast<__bf16 *>(c.opsplit.p), MT, NKS);
+    // split mode runs M2M, M2L and L2L from the fragments: no FP32 copies
+    const int64_t n = 343 * per;
+    k_split_ops<<<ceil_div(n, 256), 256, 0, c.stream>>>(c.op64, nvec, c.K, nullptr, reinterpret_cast<__bf16 *>(c.opsplit.p), MT, NKS);
     c.opsplit_ml.ensure(static_cast<size_t>(16 * 3 * per * 8));
     k_split_ops_ml<<<ceil_div(16 * per, 256), 256, 0, c.stream>>>(c.op64, c.K, reinterpret_cast<__bf16 *>(c.opsplit_ml.p), MT, NKS);
     VG_LAUNCH_CHECK();
