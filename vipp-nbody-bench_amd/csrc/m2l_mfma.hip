@@ -96,6 +96,7 @@ __global__ __launch_bounds__(64 * NG * mt_of((P + 1) * (P + 1))) void k_m2l_spli
     __shared__ uint32_t s_live[7][TB];       // bit 8 dn + qs: that source cell holds particles
     __shared__ float s_sgn[16 * NKS];        // D_qt: sign of each coefficient under this octant's mirror
     __shared__ uint4 s_msk[2 * NKS];         // the same as BF16 sign-bit masks, 8 coefficients per entry
+    __shared__ uint8_t s_nl[189];            // KS: bit n set when some target of n-tile n has a source at that step
     // XCD-aware placement: blocks go round-robin to the 8 XCDs, so XCD x takes
     // the contiguous run of work items [x per, (x+1) per); the 8 octant blocks
     // of a parent chunk, which read the same source multipoles and operators,
@@ -165,6 +166,20 @@ __global__ __launch_bounds__(64 * NG * mt_of((P + 1) * (P + 1))) void k_m2l_spli
         for (int w = 0; w < 7; ++w) s_live[w][t] = live[w];
     }
     __syncthreads();
+    if constexpr (KS) {  // n-tiles without any source at a step skip its products (their B is zero)
+        for (int st = wid; st < 189; st += NTH / 64) {
+            const int bit = 8 * s_dn[st] + s_qs[st];
+            uint32_t m = 0;
+            for (int h = 0; h < NT / 2; ++h) {
+                const int t = 64 * h + lane;
+                const uint64_t b = __ballot((s_live[bit >> 5][t] >> (bit & 31)) & 1u);
+                m |= (b & 0xffffffffull ? 1u : 0u) << (2 * h);
+                m |= (b >> 32 ? 1u : 0u) << (2 * h + 1);
+            }
+            if (lane == 0) s_nl[st] = static_cast<uint8_t>(m);
+        }
+        __syncthreads();
+    }
     // source cell of target t at step st (-1: none or empty); st is uniform
     auto source_of = [&](int st, int t) -> int32_t {
         const int dn = __builtin_amdgcn_readfirstlane(s_dn[st]), qs = __builtin_amdgcn_readfirstlane(s_qs[st]);
@@ -263,6 +278,7 @@ __global__ __launch_bounds__(64 * NG * mt_of((P + 1) * (P + 1))) void k_m2l_spli
 #pragma unroll
             for (int i = 0; i < NSO; ++i) so[i] = __builtin_amdgcn_readfirstlane(st + i < 189 ? s_o[st + i] : 0);
             asm volatile("" : "+v"(lb));  // recompute the B offsets per step rather than hold them all
+            const uint32_t nl = __builtin_amdgcn_readfirstlane(s_nl[st]);
 #pragma unroll
             for (int j = 0; j < NJ; ++j) {
                 const int h = j < JH0 ? 0 : 1, jl = j - h * JH0, jh = h ? NJ - JH0 : JH0;
@@ -291,15 +307,14 @@ __global__ __launch_bounds__(64 * NG * mt_of((P + 1) * (P + 1))) void k_m2l_spli
                 bf16x8 bh[NW], bl[NW];
                 if (active) bfrag(bf, ksl, bh, bl);
                 __builtin_amdgcn_sched_barrier(0);
-                if (!last || kw == 0) {
 #pragma unroll
-                    for (int n = 0; n < NW; ++n) acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ch, bh[n], acc[n], 0, 0, 0);
-#pragma unroll
-                    for (int n = 0; n < NW; ++n) acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ch, bl[n], acc[n], 0, 0, 0);
-                }
-                if (!last || kw == 1) {
-#pragma unroll
-                    for (int n = 0; n < NW; ++n) acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(cl, bh[n], acc[n], 0, 0, 0);
+                for (int n = 0; n < NW; ++n) {
+                    if (!((nl >> n) & 1u)) continue;
+                    if (!last || kw == 0) {
+                        acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ch, bh[n], acc[n], 0, 0, 0);
+                        acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ch, bl[n], acc[n], 0, 0, 0);
+                    }
+                    if (!last || kw == 1) acc[n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(cl, bh[n], acc[n], 0, 0, 0);
                 }
                 __builtin_amdgcn_sched_barrier(0);
                 if (jl + 1 == jh) {

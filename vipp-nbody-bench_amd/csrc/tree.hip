@@ -511,7 +511,9 @@ void tree_build(Ctx &c) {
     //    (bits below the deepest level do not matter: leaves are re-sorted by
     //    caller index); a tree that needs a deeper level is rebuilt with all
     //    21 digits sorted.
-    constexpr int SHALLOW = 16;
+    // Levels sorted: two beyond the previous tree's depth (consecutive
+    // evaluations differ little), 16 for the first tree.
+    const int SHALLOW = t.depth > 0 ? std::min(MAXD, t.depth + 2) : 16;
     int64_t ncells = 1;
     uint32_t hbuf[2];
     for (int b = 0; b < 2; ++b) {
