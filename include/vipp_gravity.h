@@ -35,7 +35,8 @@ extern "C" {
 /* Arithmetic of the force evaluation.  The RK4 state (x, v) is FP64 in every mode. */
 #define VIPP_GRAV_F64 0         /* everything FP64: parity mode against the reference's double FMM */
 #define VIPP_GRAV_F32 1         /* FP32 P2P, FP32 expansions and plain FP32 M2L */
-#define VIPP_GRAV_F32_SPLIT 2   /* FP32 P2P; M2L on MFMA with BF16 hi/lo split operands, FP32 accumulation */
+#define VIPP_GRAV_F32_SPLIT 2   /* FP32 P2P; M2L on MFMA with BF16 hi/lo split operands, M2M and L2L with a
+                                   three-way BF16 split, FP32 accumulation */
 
 /* Flags */
 #define VIPP_GRAV_FLAG_REF_FAR_G 1u  /* omit G from the far field exactly as the reference does (fmm.hpp:234,370-372) */

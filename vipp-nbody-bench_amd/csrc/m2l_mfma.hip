@@ -1,4 +1,5 @@
-// m2l_mfma.hip - M2L on the CDNA4 matrix cores with split BF16 operands.
+// m2l_mfma.hip - M2L, and in split mode M2M and L2L, on the CDNA4 matrix
+// cores with split BF16 operands.
 //
 // The M2L of one target is L~(t) = sum_o T_o M~(s(t, o)) over its (at most
 // 189) source offsets o (fmm.hpp:277-304 in the scaled real basis of
@@ -6,15 +7,16 @@
 // target sees the same 189 offsets in the same order (offsets follow from
 // (parent-neighbour offset, source octant, qt) alone).  Per offset the block
 // forms T_o (K x K) times the targets' source multipoles (K x 32 NT) on
-// v_mfma_f32_32x32x16_bf16, one wavefront per 32 output rows, accumulating
+// v_mfma_f32_32x32x16_bf16, two wavefronts per 32 output rows, accumulating
 // in FP32 across all offsets.  Every block walks the offsets in the same
 // order, so the operators stream through L2 shared by the blocks of an XCD.
 //
-// Precision: every operand x is split as hi = bf16(x), lo = bf16(x - hi)
+// Precision: every M2L operand x is split as hi = bf16(x), lo = bf16(x - hi)
 // (16 significant bits together) and a product is formed as
 // hi*hi + hi*lo + lo*hi with FP32 accumulation; the dropped lo*lo term and the
 // split residuals are ~2^-18 of the product.  T_o is split from its FP64
-// value; the multipoles are FP32 and are split in registers.
+// value; the multipoles are split once per evaluation (k_split_m).  M2M and
+// L2L use a three-way split (below).
 //
 // Operand layouts (checked on the device): for v_mfma_f32_32x32x16_bf16 lane l
 // holds A[l%32][8(l/32)+e] and B[8(l/32)+e][l%32], e = 0..7, and
