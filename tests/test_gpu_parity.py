@@ -96,6 +96,22 @@ def test_tree_deep_cluster(vg, orc):
     assert_lists_equal(l, rl)
 
 
+def test_tree_deeper_than_previous(vg, orc):
+    """A context sorts the code digits of two levels beyond its previous tree.
+    A shallow tree, then a deep one, then the shallow one again in the same
+    context: each must equal the oracle's tree bit for bit (the deep one
+    through the full re-sort)."""
+    rng = np.random.default_rng(5)
+    shallow = rng.random((1400, 3))
+    deep = np.concatenate([rng.random((1000, 3)), 0.3 + 3e-6 * rng.random((400, 3))])
+    with vg.Gravity(1400, 64, 4, "f64") as g:
+        for x in (shallow, deep, shallow):
+            g.upload(x)
+            rt, rl = orc.tree_and_lists(x, 64)
+            assert_tree_equal(g.tree(), rt)
+            assert_lists_equal(g.lists(), rl)
+
+
 def test_tree_plummer_16k(vg, orc):
     x, _, _ = orc.problem_initialize(16384, 0, 1e-3)
     t, l = gpu_tree(vg, x, 64)
@@ -198,6 +214,23 @@ def test_fmm_split_vs_oracle(vg, orc, p):
         a = gr.accelerations()
     err = np.linalg.norm(a - ref, axis=1) / np.linalg.norm(ref, axis=1)
     assert np.median(err) < 2e-6 and np.percentile(err, 99) < 1e-4, (np.median(err), np.percentile(err, 99))
+
+
+@pytest.mark.parametrize("mode", ["f32", "split"])
+def test_uniform_mass_path(vg, orc, mode):
+    """With no per-particle masses the P2P kernels leave the mass out of the
+    pair terms and scale the sums once; uploading the same mass for every
+    particle takes the general path.  The two agree to FP32 rounding: median
+    1e-6 and at most 1e-4 of each |a| (a missing or doubled mass would be off
+    by orders of magnitude)."""
+    x, _, _ = orc.problem_initialize(20000, 9, 1e-3)
+    out = []
+    for masses in (None, np.full(len(x), 1.0 / len(x))):
+        with vg.Gravity(len(x), 64, 8, mode, eps=1e-3) as gr:
+            gr.upload(x, masses=masses)
+            out.append(gr.accelerations())
+    rel = np.linalg.norm(out[0] - out[1], axis=1) / np.linalg.norm(out[1], axis=1)
+    assert rel.max() < 1e-4 and np.median(rel) < 1e-6, (rel.max(), np.median(rel))
 
 
 def test_split_m2l_locals_match_plain(vg, orc):
