@@ -12,7 +12,8 @@
 // is cut into chunks of at most P2P_CHUNK sources and each (leaf, chunk) pair
 // is one 256-thread block, so leaves with long inherited lists spread over
 // many blocks.  A block flattens its ranges in LDS, streams the chunk's sources
-// through LDS tiles and gives each lane T = 1..3 targets by leaf-size class.
+// through LDS tiles and gives each lane T = 1..6 targets, in groups of 8 to 64
+// lanes, by leaf-size class (the fewest idle lanes).
 // A leaf with one chunk writes its result; split leaves write partial sums
 // that a reduce kernel adds in chunk order (deterministic).  With a nonzero
 // softening the i == j term is exactly zero (dx = dy = dz = 0), so it needs no
@@ -25,6 +26,15 @@ namespace {
 constexpr int P2P_THREADS = 256;
 constexpr int MAXE = 27 * (MAXD + 1);
 constexpr uint32_t P2P_CHUNK = 8192;  // sources per work item
+
+// Lane layout of a leaf with nt targets and T targets per lane: groups of
+// TPW lanes, the smallest TPW in {8, 16, 32, 64} with TPW T >= nt (0: none).
+__host__ __device__ inline int tpw_for(uint32_t nt, int T) {
+    for (int tp = 8; tp <= 64; tp *= 2)
+        if (static_cast<uint32_t>(tp * T) >= nt) return tp;
+    return 0;
+}
+constexpr int TMAX = 6;  // targets per lane of the largest regular class
 
 template <class R>
 struct Vec4;
@@ -50,7 +60,7 @@ __device__ __forceinline__ void p2p_leaf(V4 *tile, V4 *red, const uint32_t *ent_
                                          V4 *__restrict__ out) {
     constexpr int CH = sizeof(R) == 4 ? 1024 : 512;
     const int tid = threadIdx.x;
-    const int TPW = nt > 32 ? 64 : (nt > 16 ? 32 : 16);
+    const int TPW = tpw_for(nt, T);  // nt <= 64 T here
     const int lane = tid & 63, wave = tid >> 6;
     const int groups = 64 / TPW;
     const int nsl = 4 * groups;
@@ -162,12 +172,15 @@ __global__ __launch_bounds__(P2P_THREADS) void k_p2p(const int2 *__restrict__ it
                                                      typename Vec4<R>::T *__restrict__ partial, uint32_t pstride) {
     using V4 = typename Vec4<R>::T;
     constexpr int CH = sizeof(R) == 4 ? 1024 : 512;
-    __shared__ V4 tile[CH];
+    constexpr int TR = T <= TMAX ? T : 4;  // T = TMAX + 1: leaves in batches of 256 targets, 4 per lane
+    // source tiles while streaming, per-lane partial sums after (a barrier apart)
+    __shared__ V4 buf[CH > P2P_THREADS * TR ? CH : P2P_THREADS * TR];
+    V4 *tile = buf;
     __shared__ uint32_t ent_start[MAXE];
     __shared__ uint32_t ent_off[MAXE + 1];
     __shared__ int32_t an[MAXD + 1];
     __shared__ int32_t an_off[MAXD + 2];
-    __shared__ V4 red[P2P_THREADS * T];
+    V4 *red = buf;
 
     const int2 item = items[blockIdx.x];  // (leaf index, chunk)
     const int32_t c = leaves[item.x];
@@ -214,9 +227,9 @@ __global__ __launch_bounds__(P2P_THREADS) void k_p2p(const int2 *__restrict__ it
     const uint32_t total = ent_off[E];
     const uint32_t src0 = static_cast<uint32_t>(item.y) * P2P_CHUNK, src1 = min(total, src0 + P2P_CHUNK);
     V4 *out = nchunk[item.x] > 1 ? partial + static_cast<size_t>(item_off[item.x] + item.y) * pstride : nullptr;
-    if (T < 4 || nt <= 256) {
+    if constexpr (T <= TMAX) {
         p2p_leaf<R, SELF_GUARD, POT, UM, T>(tile, red, ent_start, ent_off, E, src0, src1, nt, pt0, xs, eps2, G, mu, acc, out);
-    } else {  // leaves above 256 points: batches of 256 targets, each streaming the sources again
+    } else {  // leaves above 64 TMAX points: batches of 256 targets, each streaming the sources again
         for (uint32_t tb = 0; tb < nt; tb += 256) {
             p2p_leaf<R, SELF_GUARD, POT, UM, 4>(tile, red, ent_start, ent_off, E, src0, src1, min(nt - tb, 256u), pt0 + tb, xs, eps2, G, mu, acc,
                                                 out ? out + tb : nullptr);
@@ -239,8 +252,25 @@ __global__ __launch_bounds__(256) void k_ni_local(int64_t ncells, const uint32_t
     local[c] = s;
 }
 
-constexpr int NSC = 6;  // scanned per leaf: chunks, chunks of class 0..3, split flag
-__device__ inline int leaf_class(uint32_t nt) { return nt <= 64 ? 0 : (nt <= 128 ? 1 : (nt <= 192 ? 2 : 3)); }
+// Leaf classes: class T - 1 (T = 1 .. TMAX) takes T targets per lane in groups
+// of tpw_for(nt, T) lanes, T chosen for the fewest lane slots TPW T >= nt
+// (ties: fewer targets per lane); class TMAX: more than 64 TMAX targets, in
+// batches of 256.
+constexpr int NCLS = TMAX + 1;
+constexpr int NSC = NCLS + 2;  // scanned per leaf: chunks, chunks of each class, split flag
+__device__ inline int leaf_class(uint32_t nt) {
+    if (nt > 64u * TMAX) return TMAX;
+    int best = 0;
+    uint32_t slots = ~0u;
+    for (int T = 1; T <= TMAX; ++T) {
+        const int tp = tpw_for(nt, T);
+        if (tp && static_cast<uint32_t>(tp * T) < slots) {
+            slots = tp * T;
+            best = T - 1;
+        }
+    }
+    return best;
+}
 
 // Per leaf: source count -> chunks and class; block-local exclusive scans of
 // the NSC components (one leaf per thread, 256 per block) and block totals.
@@ -250,7 +280,7 @@ __global__ __launch_bounds__(256) void k_p2p_plan(const int32_t *__restrict__ le
     __shared__ uint32_t wtot[4][NSC];
     const int k = blockIdx.x * 256 + threadIdx.x;
     const unsigned lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    uint32_t v[NSC] = {0, 0, 0, 0, 0, 0};
+    uint32_t v[NSC] = {};
     if (k < nleaves) {
         const int32_t c = leaves[k];
         uint32_t src = 0;
@@ -259,7 +289,7 @@ __global__ __launch_bounds__(256) void k_p2p_plan(const int32_t *__restrict__ le
         nchunk[k] = nch;
         v[0] = nch;
         v[1 + leaf_class(count[c])] = nch;
-        v[5] = nch > 1 ? 1u : 0u;
+        v[NSC - 1] = nch > 1 ? 1u : 0u;
     }
     uint32_t inc[NSC];
 #pragma unroll
@@ -299,13 +329,13 @@ __global__ __launch_bounds__(64) void k_p2p_plan_blocks(uint32_t *__restrict__ b
         tot[q] = run;
     }
     __syncthreads();
-    if (q == 0) {
-        scal[8] = tot[0];
-        scal[9] = tot[5];
+    if (q == 0) {  // scal: [class items] [class bases] total items, split leaves
+        scal[2 * NCLS] = tot[0];
+        scal[2 * NCLS + 1] = tot[NSC - 1];
         uint32_t b = 0;
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < NCLS; ++j) {
             scal[j] = tot[1 + j];
-            scal[4 + j] = b;
+            scal[NCLS + j] = b;
             b += tot[1 + j];
         }
     }
@@ -324,9 +354,9 @@ __global__ __launch_bounds__(256) void k_p2p_items(const int32_t *__restrict__ l
     const uint32_t nch = nchunk[k];
     item_off[k] = bk[0] + pk[0];
     const int cls = leaf_class(count[leaves[k]]);
-    const uint32_t pos = scal[4 + cls] + bk[1 + cls] + pk[1 + cls];
+    const uint32_t pos = scal[NCLS + cls] + bk[1 + cls] + pk[1 + cls];
     for (uint32_t q = 0; q < nch; ++q) items[pos + q] = make_int2(k, static_cast<int>(q));
-    if (nch > 1) split_list[bk[5] + pk[5]] = k;
+    if (nch > 1) split_list[bk[NSC - 1] + pk[NSC - 1]] = k;
 }
 
 // Sum the chunk partials of split leaves in chunk order.
@@ -388,11 +418,12 @@ void p2p_t(Ctx &c, bool potential) {
     k_p2p_plan<<<nb, 256, 0, c.stream>>>(t.leaf_ids, nl, t.level, t.count, t.anc, c.p2p_local, c.p2p_nchunk, c.p2p_pre, c.p2p_btot);
     k_p2p_plan_blocks<<<1, 64, 0, c.stream>>>(c.p2p_btot, nb, c.p2p_scal);
     VG_LAUNCH_CHECK();
+    static_assert(2 * NCLS + 2 <= 16, "plan scalars");
     uint32_t hs[16];
     VG_HIP(hipMemcpyAsync(hs, c.p2p_scal.p, sizeof(hs), hipMemcpyDeviceToHost, c.stream));
     VG_HIP(hipStreamSynchronize(c.stream));
-    const uint32_t nitems = hs[8], nsplit = hs[9];
-    const uint32_t *hbase = hs + 4;
+    const uint32_t nitems = hs[2 * NCLS], nsplit = hs[2 * NCLS + 1];
+    const uint32_t *hbase = hs + NCLS;
     c.p2p_items.ensure(std::max<uint32_t>(1, nitems));
     int2 *items = reinterpret_cast<int2 *>(c.p2p_items.p);
     k_p2p_items<<<nb, 256, 0, c.stream>>>(t.leaf_ids, nl, t.count, c.p2p_nchunk, c.p2p_pre, c.p2p_btot, c.p2p_scal, c.p2p_item_off, items, c.p2p_split);
@@ -409,7 +440,7 @@ void p2p_t(Ctx &c, bool potential) {
             partial = reinterpret_cast<V4 *>(c.p2p_part64.p);
         }
     }
-    for (int cls = 0; cls < 4; ++cls) {
+    for (int cls = 0; cls < NCLS; ++cls) {
         if (!hs[cls]) continue;
         const dim3 grid(hs[cls]);
         const int2 *it = items + hbase[cls];
@@ -424,8 +455,14 @@ void p2p_t(Ctx &c, bool potential) {
             VG_P2P_T(G_, P_, U_, 2); \
         else if (cls == 2)           \
             VG_P2P_T(G_, P_, U_, 3); \
-        else                         \
+        else if (cls == 3)           \
             VG_P2P_T(G_, P_, U_, 4); \
+        else if (cls == 4)           \
+            VG_P2P_T(G_, P_, U_, 5); \
+        else if (cls == 5)           \
+            VG_P2P_T(G_, P_, U_, 6); \
+        else /* batches */           \
+            VG_P2P_T(G_, P_, U_, 7); \
     } while (0)
         if (guard) {
             if (potential)
