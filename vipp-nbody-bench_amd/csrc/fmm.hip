@@ -508,6 +508,10 @@ __global__ __launch_bounds__(64) void k_l2p(const int32_t *__restrict__ leaves, 
     const double iw = 1.0 / w;
     const double cx = (bmin[3 * c] + bmax[3 * c]) / 2, cy = (bmin[3 * c + 1] + bmax[3 * c + 1]) / 2, cz = (bmin[3 * c + 2] + bmax[3 * c + 2]) / 2;
     for (uint32_t i = threadIdx.x; i < cnt; i += 64) {
+        // read the local coefficients from LDS inside the loop rather than hold all of them in registers
+        int z0 = 0;
+        asm volatile("" : "+v"(z0));
+        const R *slv = sl + z0;
         const uint32_t k = s + i;
         const R x = static_cast<R>((xs64[4 * k] - cx) * iw), y = static_cast<R>((xs64[4 * k + 1] - cy) * iw), z = static_cast<R>((xs64[4 * k + 2] - cz) * iw);
         const R r2 = x * x + y * y + z * z;
@@ -534,25 +538,25 @@ __global__ __launch_bounds__(64) void k_l2p(const int32_t *__restrict__ leaves, 
                 }
                 // potential: sum L_n^0 R_n^0 + 2 Re(L_n^m R_n^m)
                 if (m == 0)
-                    phi += sl[kidx_re(n, 0)] * cre;
+                    phi += slv[kidx_re(n, 0)] * cre;
                 else
-                    phi += 2 * (sl[kidx_re(n, m)] * cre - sl[kidx_im(n, m)] * cim);
+                    phi += 2 * (slv[kidx_re(n, m)] * cre - slv[kidx_im(n, m)] * cim);
                 if (n + 1 <= P) {
                     const int l = n + 1;  // R_n^m enters the derivatives of degree-l terms
                     const R lz = static_cast<R>(k_lz(l, m));
                     if (m == 0)
-                        gz += sl[kidx_re(l, 0)] * lz * cre;
+                        gz += slv[kidx_re(l, 0)] * lz * cre;
                     else
-                        gz += 2 * lz * (sl[kidx_re(l, m)] * cre - sl[kidx_im(l, m)] * cim);
+                        gz += 2 * lz * (slv[kidx_re(l, m)] * cre - slv[kidx_im(l, m)] * cim);
                     {  // -sqrt((l+m+1)(l+m)) L_l^{m+1} R_n^m
                         const R la = static_cast<R>(k_la(l, m + 1));
-                        const R Lre = sl[kidx_re(l, m + 1)], Lim = sl[kidx_im(l, m + 1)];
+                        const R Lre = slv[kidx_re(l, m + 1)], Lim = slv[kidx_im(l, m + 1)];
                         gre -= la * (Lre * cre - Lim * cim);
                         gim -= la * (Lre * cim + Lim * cre);
                     }
                     if (m >= 1) {  // sqrt((l-m+1)(l-m)) conj(L_l^{m-1} R_n^m), with Re(L_l^0) for m = 1
                         const R lb = static_cast<R>(k_lb(l, m - 1));
-                        const R Lre = sl[kidx_re(l, m - 1)], Lim = (m - 1) == 0 ? R(0) : sl[kidx_im(l, m - 1)];
+                        const R Lre = slv[kidx_re(l, m - 1)], Lim = (m - 1) == 0 ? R(0) : slv[kidx_im(l, m - 1)];
                         gre += lb * (Lre * cre - Lim * cim);
                         gim -= lb * (Lre * cim + Lim * cre);
                     }
