@@ -223,6 +223,13 @@ __global__ __launch_bounds__(256) void k_p2m(const int32_t *__restrict__ leaves,
     const uint32_t s = start[c], cnt = count[c];
     const double iw = 1.0 / ldexp(wx0, -level[c]);
     const double cx = (bmin[3 * c] + bmax[3 * c]) / 2, cy = (bmin[3 * c + 1] + bmax[3 * c + 1]) / 2, cz = (bmin[3 * c + 2] + bmax[3 * c + 2]) / 2;
+    // R_m^m = (prod_{q<=m} cDiag[q]) (x + i y)^m: the products for m1 and m2
+    double sc1 = 1.0, sc2 = 1.0;
+    for (int q = 1; q <= m2; ++q) {
+        if (q <= m1) sc1 *= cDiag[q];
+        sc2 *= cDiag[q];
+    }
+    const R s1 = static_cast<R>(sc1), s2 = static_cast<R>(sc2);
     // recurrence coefficients of each step (unused at the two chain starts)
     R c1[NS], c2[NS], are[NS], aim[NS];
 #pragma unroll
@@ -241,22 +248,28 @@ __global__ __launch_bounds__(256) void k_p2m(const int32_t *__restrict__ leaves,
                     z = static_cast<R>((xs64[4 * k + 2] - cz) * iw);
             const R mi = static_cast<R>(xs64[4 * k + 3]);
             const R r2 = x * x + y * y + z * z;
-            // diagonal R_q^q, q = 1..p (the same for every lane), keeping q = m1 and q = m2
-            R dre = 1, dim = 0, d1re = 1, d1im = 0, d2re = 1, d2im = 0;
-            for (int q = 1; q <= p; ++q) {
-                const R d = static_cast<R>(cDiag[q]);
-                const R nre = d * (x * dre - y * dim), nim = d * (x * dim + y * dre);
-                dre = nre;
-                dim = nim;
-                if (q == m1) {
-                    d1re = dre;
-                    d1im = dim;
-                }
-                if (q == m2) {
-                    d2re = dre;
-                    d2im = dim;
+            // (x + i y)^m1 and ^m2 by repeated squaring: w^(2^b) once, a
+            // multiply per set bit of each exponent (selects, no branches)
+            R d1re = 1, d1im = 0, d2re = 1, d2im = 0;
+            {
+                R wre = x, wim = y;  // w^(2^b)
+                for (int b = 0; (1 << b) <= p; ++b) {
+                    const R t1re = d1re * wre - d1im * wim, t1im = d1re * wim + d1im * wre;
+                    const R t2re = d2re * wre - d2im * wim, t2im = d2re * wim + d2im * wre;
+                    const bool u1 = (m1 >> b) & 1, u2 = (m2 >> b) & 1;
+                    d1re = u1 ? t1re : d1re;
+                    d1im = u1 ? t1im : d1im;
+                    d2re = u2 ? t2re : d2re;
+                    d2im = u2 ? t2im : d2im;
+                    const R sre = wre * wre - wim * wim, sim = 2 * wre * wim;
+                    wre = sre;
+                    wim = sim;
                 }
             }
+            d1re *= s1;
+            d1im *= s1;
+            d2re *= s2;
+            d2im *= s2;
             R bre = 0, bim = 0, cre = 0, cim = 0;  // R_{n-2}^m, R_{n-1}^m
 #pragma unroll
             for (int st = 0; st < NS; ++st) {
