@@ -273,14 +273,14 @@ def learned_comparison(vgm, workloads, args):
     low, a4 = run(4, "split", False)
     learned, ac = run(4, "split", True, bound)
     learned_nofb, acn = run(4, "split", True)
-    judge = lambda a: float(np.median(np.linalg.norm(a[idx] - a12[idx], axis=1) / np.linalg.norm(a12[idx], axis=1)))
+    diff_to_p12 = lambda a: float(np.median(np.linalg.norm(a[idx] - a12[idx], axis=1) / np.linalg.norm(a12[idx], axis=1)))
     return {
         "workload": wl, "n": n, "train_pairs": int(len(X)), "train_seeds": [1, 2, 3], "eval_seed": 0,
         "pairs_s": t1 - t0, "train_s": t2 - t1, "train_loss_first_last": [float(hist[0]), float(hist[-1])],
         "fallback_fraction_target": args.fallback_fraction, "bound": bound,
         "plain_fp32_p12": plain, "split_p12": split12, "p4_no_closure": low, "p4_closure": learned_nofb, "p4_closure_fallback": learned,
         "speedup_vs_plain": learned["steps_per_s"] / plain["steps_per_s"],
-        "median_diff_to_p12": {"p4": judge(a4), "p4_closure": judge(acn), "p4_closure_fallback": judge(ac)},
+        "median_diff_to_p12": {"p4": diff_to_p12(a4), "p4_closure": diff_to_p12(acn), "p4_closure_fallback": diff_to_p12(ac)},
         "err_sample": int(len(idx)), "err_reference": "fp64 direct sum on the device",
     }
 

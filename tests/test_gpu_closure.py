@@ -150,7 +150,7 @@ def _rel(a, b):
 
 def test_closure_gain_over_plain_low_order(vg, cl, orc, trained):
     """Ablation: a p = 4 FMM with the trained network against the same p = 4 FMM
-    without it, both judged against the order-12 FMM, on positions the network
+    without it, both compared with the order-12 FMM, on positions the network
     never saw.  The network must shrink the median difference to order 12."""
     params, io, hist, _ = trained
     assert hist[-1] < 0.8 * hist[0]
