@@ -143,7 +143,7 @@ struct Ctx {
 
     // P2P work items
     DevBuf<uint32_t> p2p_scal, p2p_nchunk, p2p_item_off, p2p_local, p2p_pre, p2p_btot;
-    DevBuf<int32_t> p2p_split;  // leaves whose sources span several chunks
+    DevBuf<uint32_t> p2p_ticket;  // per leaf: work items of a split leaf that have finished
     DevBuf<int64_t> p2p_items;  // int2 (leaf index, chunk)
     DevBuf<float> p2p_part32;
     DevBuf<double> p2p_part64;

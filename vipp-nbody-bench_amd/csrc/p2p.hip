@@ -14,8 +14,9 @@
 // many blocks.  A block flattens its ranges in LDS, streams the chunk's sources
 // through LDS tiles and gives each lane T = 1..6 targets, in groups of 8 to 64
 // lanes, by leaf-size class (the fewest idle lanes).
-// A leaf with one chunk writes its result; split leaves write partial sums
-// that a reduce kernel adds in chunk order (deterministic).  With a nonzero
+// A leaf with one chunk writes its result; split leaves write partial sums,
+// and the work item that finishes last adds them in chunk order
+// (deterministic).  With a nonzero
 // softening the i == j term is exactly zero (dx = dy = dz = 0), so it needs no
 // branch; without softening a select removes it.
 #include "gravity.hpp"
@@ -169,7 +170,7 @@ __global__ __launch_bounds__(P2P_THREADS) void k_p2p(const int2 *__restrict__ it
                                                      const uint32_t *__restrict__ start, const uint32_t *__restrict__ count, const int32_t *__restrict__ anc,
                                                      const int32_t *__restrict__ ni, const int32_t *__restrict__ ni_cnt,
                                                      const typename Vec4<R>::T *__restrict__ xs, R eps2, R G, R mu, typename Vec4<R>::T *__restrict__ acc,
-                                                     typename Vec4<R>::T *__restrict__ partial, uint32_t pstride) {
+                                                     typename Vec4<R>::T *__restrict__ partial, uint32_t pstride, uint32_t *__restrict__ ticket) {
     using V4 = typename Vec4<R>::T;
     constexpr int CH = sizeof(R) == 4 ? 1024 : 512;
     constexpr int TR = T <= TMAX ? T : 4;  // T = TMAX + 1: leaves in batches of 256 targets, 4 per lane
@@ -234,6 +235,32 @@ __global__ __launch_bounds__(P2P_THREADS) void k_p2p(const int2 *__restrict__ it
             p2p_leaf<R, SELF_GUARD, POT, UM, 4>(tile, red, ent_start, ent_off, E, src0, src1, min(nt - tb, 256u), pt0 + tb, xs, eps2, G, mu, acc,
                                                 out ? out + tb : nullptr);
             __syncthreads();
+        }
+    }
+    if (out) {
+        // A leaf split over several work items: the block that finishes last
+        // adds the partial sums in chunk order (the same order every run).
+        __shared__ uint32_t s_last;
+        const uint32_t nch = nchunk[item.x];
+        __threadfence();  // this block's partial sums are visible device-wide
+        __syncthreads();
+        if (tid == 0) s_last = atomicAdd(&ticket[item.x], 1u) == nch - 1 ? 1u : 0u;
+        __syncthreads();
+        if (s_last) {
+            __threadfence();  // and the other blocks' ones to this block
+            const V4 *pk = partial + static_cast<size_t>(item_off[item.x]) * pstride;
+            for (uint32_t t = tid; t < nt; t += P2P_THREADS) {
+                R sx = 0, sy = 0, sz = 0, sp = 0;
+                for (uint32_t q = 0; q < nch; ++q) {
+                    const V4 v = pk[static_cast<size_t>(q) * pstride + t];
+                    sx += v.x;
+                    sy += v.y;
+                    sz += v.z;
+                    sp += v.w;
+                }
+                if (POT && !SELF_GUARD) sp -= xs[pt0 + t].w * rsq(eps2);
+                acc[pt0 + t] = V4{G * sx, G * sy, G * sz, sp};
+            }
         }
     }
 }
@@ -313,30 +340,38 @@ __global__ __launch_bounds__(256) void k_p2p_plan(const int32_t *__restrict__ le
 }
 
 // One block: exclusive scan of the block totals; writes the class counts and
-// bases, the item total and the split-leaf total into scal.
-//   scal[0..3] items of class 0..3, scal[4..7] class bases, scal[8] items,
-//   scal[9] split leaves.
-__global__ __launch_bounds__(64) void k_p2p_plan_blocks(uint32_t *__restrict__ btot, int nblocks, uint32_t *__restrict__ scal) {
+// bases, the item total and the split-leaf total into scal:
+//   scal[0..NCLS) items per class, scal[NCLS..2 NCLS) class bases,
+//   scal[2 NCLS] items, scal[2 NCLS + 1] split leaves.
+__global__ __launch_bounds__(256) void k_p2p_plan_blocks(uint32_t *__restrict__ btot, int nblocks, uint32_t *__restrict__ scal) {
+    // exclusive scan of each component over the plan blocks, one wavefront per
+    // component (wavefront-wide inclusive scans, carried across runs of 64)
     __shared__ uint32_t tot[NSC];
-    const int q = threadIdx.x;
-    if (q < NSC) {
-        uint32_t run = 0;
-        for (int b = 0; b < nblocks; ++b) {
-            const uint32_t t = btot[b * NSC + q];
-            btot[b * NSC + q] = run;
-            run += t;
+    const unsigned lane = threadIdx.x & 63;
+    for (int q = threadIdx.x >> 6; q < NSC; q += 4) {
+        uint32_t carry = 0;
+        for (int b0 = 0; b0 < nblocks; b0 += 64) {
+            const int b = b0 + static_cast<int>(lane);
+            const uint32_t v = b < nblocks ? btot[b * NSC + q] : 0u;
+            uint32_t x = v;
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t o = __shfl_up(x, d, 64);
+                if (lane >= static_cast<unsigned>(d)) x += o;
+            }
+            if (b < nblocks) btot[b * NSC + q] = carry + x - v;
+            carry += __shfl(x, 63, 64);
         }
-        tot[q] = run;
+        if (lane == 0) tot[q] = carry;
     }
     __syncthreads();
-    if (q == 0) {  // scal: [class items] [class bases] total items, split leaves
+    if (threadIdx.x == 0) {  // scal: [class items] [class bases] total items, split leaves
         scal[2 * NCLS] = tot[0];
         scal[2 * NCLS + 1] = tot[NSC - 1];
-        uint32_t b = 0;
+        uint32_t base = 0;
         for (int j = 0; j < NCLS; ++j) {
             scal[j] = tot[1 + j];
-            scal[NCLS + j] = b;
-            b += tot[1 + j];
+            scal[NCLS + j] = base;
+            base += tot[1 + j];
         }
     }
 }
@@ -346,7 +381,7 @@ __global__ __launch_bounds__(64) void k_p2p_plan_blocks(uint32_t *__restrict__ b
 __global__ __launch_bounds__(256) void k_p2p_items(const int32_t *__restrict__ leaves, int nleaves, const uint32_t *__restrict__ count,
                                                    const uint32_t *__restrict__ nchunk, const uint32_t *__restrict__ pre,
                                                    const uint32_t *__restrict__ boff, const uint32_t *__restrict__ scal,
-                                                   uint32_t *__restrict__ item_off, int2 *__restrict__ items, int32_t *__restrict__ split_list) {
+                                                   uint32_t *__restrict__ item_off, int2 *__restrict__ items) {
     const int k = blockIdx.x * 256 + threadIdx.x;
     if (k >= nleaves) return;
     const uint32_t *pk = pre + static_cast<int64_t>(k) * NSC;
@@ -356,33 +391,6 @@ __global__ __launch_bounds__(256) void k_p2p_items(const int32_t *__restrict__ l
     const int cls = leaf_class(count[leaves[k]]);
     const uint32_t pos = scal[NCLS + cls] + bk[1 + cls] + pk[1 + cls];
     for (uint32_t q = 0; q < nch; ++q) items[pos + q] = make_int2(k, static_cast<int>(q));
-    if (nch > 1) split_list[bk[NSC - 1] + pk[NSC - 1]] = k;
-}
-
-// Sum the chunk partials of split leaves in chunk order.
-template <class R, bool SELF_GUARD, bool POT>
-__global__ __launch_bounds__(64) void k_p2p_reduce(const int32_t *__restrict__ split_list, const int32_t *__restrict__ leaves,
-                                                   const uint32_t *__restrict__ nchunk, const uint32_t *__restrict__ item_off,
-                                                   const uint32_t *__restrict__ start, const uint32_t *__restrict__ count,
-                                                   const typename Vec4<R>::T *__restrict__ xs, R eps2, R G, const typename Vec4<R>::T *__restrict__ partial,
-                                                   uint32_t pstride, typename Vec4<R>::T *__restrict__ acc) {
-    using V4 = typename Vec4<R>::T;
-    const int k = split_list[blockIdx.x];
-    const uint32_t nch = nchunk[k];
-    const int32_t c = leaves[k];
-    const uint32_t nt = count[c], pt0 = start[c];
-    for (uint32_t t = threadIdx.x; t < nt; t += 64) {
-        R sx = 0, sy = 0, sz = 0, sp = 0;
-        for (uint32_t q = 0; q < nch; ++q) {
-            const V4 v = partial[static_cast<size_t>(item_off[k] + q) * pstride + t];
-            sx += v.x;
-            sy += v.y;
-            sz += v.z;
-            sp += v.w;
-        }
-        if (POT && !SELF_GUARD) sp -= xs[pt0 + t].w * rsq(eps2);
-        acc[pt0 + t] = V4{G * sx, G * sy, G * sz, sp};
-    }
 }
 
 template <class R>
@@ -413,10 +421,10 @@ void p2p_t(Ctx &c, bool potential) {
     c.p2p_local.ensure(t.ncells);
     c.p2p_pre.ensure(static_cast<size_t>(nl) * NSC);
     c.p2p_btot.ensure(static_cast<size_t>(nb) * NSC);
-    c.p2p_split.ensure(nl);
+    c.p2p_ticket.ensure(nl);
     k_ni_local<<<ceil_div(t.ncells, 256), 256, 0, c.stream>>>(t.ncells, t.count, t.ni, t.ni_cnt, c.p2p_local);
     k_p2p_plan<<<nb, 256, 0, c.stream>>>(t.leaf_ids, nl, t.level, t.count, t.anc, c.p2p_local, c.p2p_nchunk, c.p2p_pre, c.p2p_btot);
-    k_p2p_plan_blocks<<<1, 64, 0, c.stream>>>(c.p2p_btot, nb, c.p2p_scal);
+    k_p2p_plan_blocks<<<1, 256, 0, c.stream>>>(c.p2p_btot, nb, c.p2p_scal);
     VG_LAUNCH_CHECK();
     static_assert(2 * NCLS + 2 <= 16, "plan scalars");
     uint32_t hs[16];
@@ -426,12 +434,13 @@ void p2p_t(Ctx &c, bool potential) {
     const uint32_t *hbase = hs + NCLS;
     c.p2p_items.ensure(std::max<uint32_t>(1, nitems));
     int2 *items = reinterpret_cast<int2 *>(c.p2p_items.p);
-    k_p2p_items<<<nb, 256, 0, c.stream>>>(t.leaf_ids, nl, t.count, c.p2p_nchunk, c.p2p_pre, c.p2p_btot, c.p2p_scal, c.p2p_item_off, items, c.p2p_split);
+    k_p2p_items<<<nb, 256, 0, c.stream>>>(t.leaf_ids, nl, t.count, c.p2p_nchunk, c.p2p_pre, c.p2p_btot, c.p2p_scal, c.p2p_item_off, items);
     VG_LAUNCH_CHECK();
     const bool split = nsplit > 0;
     const uint32_t pstride = c.thr;
     V4 *partial = nullptr;
     if (split) {
+        VG_HIP(hipMemsetAsync(c.p2p_ticket.p, 0, static_cast<size_t>(nl) * sizeof(uint32_t), c.stream));
         if (sizeof(R) == 4) {
             c.p2p_part32.ensure(4 * static_cast<size_t>(nitems) * pstride);
             partial = reinterpret_cast<V4 *>(c.p2p_part32.p);
@@ -446,7 +455,7 @@ void p2p_t(Ctx &c, bool potential) {
         const int2 *it = items + hbase[cls];
 #define VG_P2P_T(G_, P_, U_, T_)                                                                                                                \
     k_p2p<R, G_, P_, U_, T_><<<grid, P2P_THREADS, 0, c.stream>>>(it, t.leaf_ids, c.p2p_nchunk, c.p2p_item_off, t.level, t.start, t.count, t.anc, \
-                                                                 t.ni, t.ni_cnt, xs, eps2, static_cast<R>(c.G), mu, acc, partial, pstride)
+                                                                 t.ni, t.ni_cnt, xs, eps2, static_cast<R>(c.G), mu, acc, partial, pstride, c.p2p_ticket)
 #define VG_P2P(G_, P_, U_)           \
     do {                             \
         if (cls == 0)                \
@@ -481,23 +490,6 @@ void p2p_t(Ctx &c, bool potential) {
         }
 #undef VG_P2P
 #undef VG_P2P_T
-    }
-    if (split) {
-#define VG_RED(G_, P_) \
-    k_p2p_reduce<R, G_, P_><<<nsplit, 64, 0, c.stream>>>(c.p2p_split, t.leaf_ids, c.p2p_nchunk, c.p2p_item_off, t.start, t.count, xs, eps2, \
-                                                         static_cast<R>(c.G), partial, pstride, acc)
-        if (guard) {
-            if (potential)
-                VG_RED(true, true);
-            else
-                VG_RED(true, false);
-        } else {
-            if (potential)
-                VG_RED(false, true);
-            else
-                VG_RED(false, false);
-        }
-#undef VG_RED
     }
     VG_LAUNCH_CHECK();
 }
