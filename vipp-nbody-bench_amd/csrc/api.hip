@@ -334,6 +334,7 @@ int vipp_grav_create(const vipp_grav_config *cfg, vipp_grav_ctx **out) {
             c.device = cfg->device;
             VG_HIP(hipSetDevice(c.device));
             VG_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+            VG_HIP(hipHostMalloc(&c.pinned, vg::Ctx::PINNED_BYTES, hipHostMallocDefault));
             for (auto &e : c.ev) VG_HIP(hipEventCreate(&e));
             vg::fmm_constants_init();
             const size_t n3 = 3 * static_cast<size_t>(c.n);
@@ -360,6 +361,7 @@ int vipp_grav_destroy(vipp_grav_ctx *ctx) {
     for (auto &e : ctx->c.ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->c.stream) (void)hipStreamDestroy(ctx->c.stream);
+    if (ctx->c.pinned) (void)hipHostFree(ctx->c.pinned);
     delete ctx;
     return rc;
 }

@@ -161,9 +161,10 @@ bool closure_fallback_prepare(Ctx &c) {
     Closure &cl = c.cl;
     TreeDev &t = c.t;
     if (!(cl.bound < INFINITY) || t.nleaves == 0) return false;
-    uint32_t nf = 0;
-    VG_HIP(hipMemcpyAsync(&nf, cl.scal.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
+    uint32_t *hnf = reinterpret_cast<uint32_t *>(static_cast<char *>(c.pinned) + 256);  // a pinned host word
+    VG_HIP(hipMemcpyAsync(hnf, cl.scal.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
     VG_HIP(hipStreamSynchronize(c.stream));
+    const uint32_t nf = *hnf;
     cl.nflag = nf;
     if (nf == 0) return false;
     const int64_t n = t.nleaves;

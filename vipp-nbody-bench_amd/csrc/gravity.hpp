@@ -93,6 +93,10 @@ struct Ctx {
     std::string err;
 
     hipStream_t stream = nullptr;
+    // pinned host words for the scalars read back during an evaluation (tree
+    // level sizes, root box, P2P plan): a pinned copy returns sooner
+    void *pinned = nullptr;
+    static constexpr size_t PINNED_BYTES = 4096;
     bool timing = false;
     hipEvent_t ev[2 * PH_COUNT] = {};
     double phase_ms[PH_COUNT] = {};

@@ -427,8 +427,8 @@ void p2p_t(Ctx &c, bool potential) {
     k_p2p_plan_blocks<<<1, 256, 0, c.stream>>>(c.p2p_btot, nb, c.p2p_scal);
     VG_LAUNCH_CHECK();
     static_assert(2 * NCLS + 2 <= 16, "plan scalars");
-    uint32_t hs[16];
-    VG_HIP(hipMemcpyAsync(hs, c.p2p_scal.p, sizeof(hs), hipMemcpyDeviceToHost, c.stream));
+    uint32_t *hs = reinterpret_cast<uint32_t *>(static_cast<char *>(c.pinned) + 128);  // 16 pinned host words
+    VG_HIP(hipMemcpyAsync(hs, c.p2p_scal.p, 16 * sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
     VG_HIP(hipStreamSynchronize(c.stream));
     const uint32_t nitems = hs[2 * NCLS], nsplit = hs[2 * NCLS + 1];
     const uint32_t *hbase = hs + NCLS;

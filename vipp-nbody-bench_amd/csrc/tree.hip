@@ -515,7 +515,7 @@ void tree_build(Ctx &c) {
     // evaluations differ little), 16 for the first tree.
     const int SHALLOW = t.depth > 0 ? std::min(MAXD, t.depth + 2) : 16;
     int64_t ncells = 1;
-    uint32_t hbuf[2];
+    uint32_t *hbuf = reinterpret_cast<uint32_t *>(static_cast<char *>(c.pinned) + 64);  // pinned host words
     for (int b = 0; b < 2; ++b) {
         c.key[b].ensure(std::max<int64_t>(1, n));
         c.val[b].ensure(std::max<int64_t>(1, n));
@@ -550,7 +550,7 @@ void tree_build(Ctx &c) {
     k_root<<<1, 64, 0, st>>>(c.box, n, c.thr, t.start, t.count, t.child, t.parent, t.octant, t.level, t.bmin, t.bmax, t.grid, t.anc, t.int_ids,
                             c.scal.p);
     VG_LAUNCH_CHECK();
-    double hbox[8];
+    double *hbox = static_cast<double *>(c.pinned);  // 8 doubles
     VG_HIP(hipMemcpyAsync(hbox, c.box, 7 * sizeof(double), hipMemcpyDeviceToHost, st));
     VG_HIP(hipMemcpyAsync(hbuf, c.scal.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     VG_HIP(hipStreamSynchronize(st));
