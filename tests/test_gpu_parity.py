@@ -216,6 +216,30 @@ def test_fmm_split_vs_oracle(vg, orc, p):
     assert np.median(err) < 2e-6 and np.percentile(err, 99) < 1e-4, (np.median(err), np.percentile(err, 99))
 
 
+def test_large_leaves(vg, orc):
+    """s = 256 makes leaves of up to floor(sqrt(8) 256) = 724 points, so the
+    P2P takes its batched path (256 targets at a time) for the largest ones:
+    FP64 against the oracle (1e-11 of max |a|), FP32 against FP64 (median
+    1e-6)."""
+    x, _, _ = orc.problem_initialize(30000, 11, 1e-3)
+    ref = orc.fmm(x, 256, 4, eps=1e-3)
+    out = {}
+    for mode in ("f64", "f32"):
+        with vg.Gravity(len(x), 256, 4, mode, eps=1e-3) as gr:
+            gr.upload(x)
+            out[mode] = gr.accelerations()
+            if mode == "f64":
+                t = gr.tree()
+    cnt = np.diff(np.asarray(t.pt_off))
+    par = np.asarray(t.parent)
+    leaf = np.ones(len(cnt), bool)
+    leaf[par[par >= 0]] = False
+    assert cnt[leaf].max() > 384, cnt[leaf].max()  # some leaf takes the batched path
+    assert np.abs(out["f64"] - ref).max() <= 1e-11 * np.abs(ref).max()
+    rel = np.linalg.norm(out["f32"] - out["f64"], axis=1) / np.linalg.norm(out["f64"], axis=1)
+    assert np.median(rel) < 1e-6, np.median(rel)
+
+
 @pytest.mark.parametrize("mode", ["f32", "split"])
 def test_uniform_mass_path(vg, orc, mode):
     """With no per-particle masses the P2P kernels leave the mass out of the
