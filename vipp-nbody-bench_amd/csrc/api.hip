@@ -69,8 +69,48 @@ void fmm_m2l(Ctx &c) {
         fmm_m2l_plain(c);
 }
 
+void join_p2p(Ctx &c) {
+    if (!c.p2p_pending) return;
+    VG_HIP(hipStreamWaitEvent(c.stream, c.ev_join, 0));
+    c.p2p_pending = false;
+}
+
+// Untimed evaluations launch P2P (VALU-bound, about half the evaluation) on a
+// second stream once the tree and lists exist, so that the small launches of
+// the operators and the upward pass, and M2L (MFMA-bound), run beside it; L2P
+// waits for it. Timed evaluations keep one stream, so that each phase's events
+// bracket its own kernels only.
+static void evaluate_overlapped(Ctx &c, bool potential) {
+    VG_HIP(hipEventRecord(c.ev_fork, c.stream));
+    VG_HIP(hipStreamWaitEvent(c.stream2, c.ev_fork, 0));
+    operators_build(c);
+    fmm_upward(c);  // enqueued first: the GPU has work while the host waits for the P2P plan
+    {
+        struct Swap {
+            Ctx &c;
+            explicit Swap(Ctx &c_) : c(c_) { std::swap(c.stream, c.stream2); }
+            ~Swap() { std::swap(c.stream, c.stream2); }
+        } on_stream2(c);
+        fmm_p2p(c, potential);
+    }
+    VG_HIP(hipEventRecord(c.ev_join, c.stream2));
+    c.p2p_pending = true;
+    fmm_m2l(c);
+    fmm_downward(c, potential);
+    join_p2p(c);
+    scatter_accel(c, potential);
+}
+
 void evaluate(Ctx &c, bool potential) {
+    join_p2p(c);  // a P2P left pending by an evaluation that threw must end before the tree changes
     bool used[PH_COUNT] = {};
+    if (!c.timing && c.stream2) {
+        tree_build(c);
+        lists_build(c);
+        evaluate_overlapped(c, potential);
+        c.evaluated = true;
+        return;
+    }
     phase_begin(c, PH_TREE);
     tree_build(c);
     phase_end(c, PH_TREE);
@@ -334,6 +374,9 @@ int vipp_grav_create(const vipp_grav_config *cfg, vipp_grav_ctx **out) {
             c.device = cfg->device;
             VG_HIP(hipSetDevice(c.device));
             VG_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+            VG_HIP(hipStreamCreateWithFlags(&c.stream2, hipStreamNonBlocking));
+            VG_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
+            VG_HIP(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
             VG_HIP(hipHostMalloc(&c.pinned, vg::Ctx::PINNED_BYTES, hipHostMallocDefault));
             for (auto &e : c.ev) VG_HIP(hipEventCreate(&e));
             vg::fmm_constants_init();
@@ -357,10 +400,14 @@ int vipp_grav_destroy(vipp_grav_ctx *ctx) {
     int rc = guarded(ctx, [&] {
         VG_HIP(hipSetDevice(ctx->c.device));
         if (ctx->c.stream) VG_HIP(hipStreamSynchronize(ctx->c.stream));
+        if (ctx->c.stream2) VG_HIP(hipStreamSynchronize(ctx->c.stream2));
     });
     for (auto &e : ctx->c.ev)
         if (e) (void)hipEventDestroy(e);
+    if (ctx->c.ev_fork) (void)hipEventDestroy(ctx->c.ev_fork);
+    if (ctx->c.ev_join) (void)hipEventDestroy(ctx->c.ev_join);
     if (ctx->c.stream) (void)hipStreamDestroy(ctx->c.stream);
+    if (ctx->c.stream2) (void)hipStreamDestroy(ctx->c.stream2);
     if (ctx->c.pinned) (void)hipHostFree(ctx->c.pinned);
     delete ctx;
     return rc;

@@ -782,6 +782,7 @@ void downward_t(Ctx &c) {
         const int64_t a = t.int_off[d], b = t.int_off[d + 1];
         if (b > a) l2l_level<R>(c, t.int_ids.p + a, b - a);
     }
+    join_p2p(c);  // L2P adds to the P2P accelerations
     bool fallback = false;
     if constexpr (sizeof(R) == 4) {
         if (c.cl.on) {

@@ -93,6 +93,11 @@ struct Ctx {
     std::string err;
 
     hipStream_t stream = nullptr;
+    // P2P runs on stream2 beside the upward pass and M2L of the same
+    // evaluation (they share no buffer); L2P waits for it (join_p2p)
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    bool p2p_pending = false;
     // pinned host words for the scalars read back during an evaluation (tree
     // level sizes, root box, P2P plan): a pinned copy returns sooner
     void *pinned = nullptr;
@@ -166,6 +171,7 @@ void fmm_upward(Ctx &c);                 // P2M + M2M
 void fmm_m2l(Ctx &c);                    // M2L (plain VALU or split MFMA by mode)
 void fmm_p2p(Ctx &c, bool potential);    // near field into the sorted accelerations
 void fmm_downward(Ctx &c, bool potential);  // L2L + L2P
+void join_p2p(Ctx &c);                       // ctx.stream waits for a P2P launched on ctx.stream2
 void scatter_accel(Ctx &c, bool potential);  // sorted -> caller order
 void rk4_init(Ctx &c);
 void rk4_stage(Ctx &c, int stage, double dt);
