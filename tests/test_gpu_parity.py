@@ -112,6 +112,31 @@ def test_tree_deeper_than_previous(vg, orc):
             assert_lists_equal(g.lists(), rl)
 
 
+def test_tree_device_levels(vg, orc):
+    """From its second tree on, a context keeps the level sizes on the device,
+    in a cell capacity of twice its previous tree's cells (at least 1024).
+    A uniform tree (585 cells), then one with two dense regions (1721 cells,
+    over that capacity: built again level by level), the same again and a
+    moved copy (on the device), then the first again: each must equal the
+    oracle's tree and lists bit for bit, and the forces of the last one the
+    forces of a fresh context."""
+    n = 1 << 16
+    rng = np.random.default_rng(12)
+    a = rng.random((n, 3))
+    b = np.concatenate([rng.random((n // 2, 3)), 0.01 + 0.23 * rng.random((n // 4, 3)), 0.51 + 0.23 * rng.random((n // 4, 3))])
+    c = b + 1e-4 * rng.standard_normal(b.shape)
+    with vg.Gravity(n, 64, 4, "f64", eps=1e-3) as g:
+        for x in (a, b, b, c, a):
+            g.upload(x)
+            rt, rl = orc.tree_and_lists(x, 64)
+            assert_tree_equal(g.tree(), rt)
+            assert_lists_equal(g.lists(), rl)
+        acc = g.accelerations()
+    with vg.Gravity(n, 64, 4, "f64", eps=1e-3) as g:
+        g.upload(a)
+        assert np.array_equal(g.accelerations(), acc)
+
+
 def test_tree_plummer_16k(vg, orc):
     x, _, _ = orc.problem_initialize(16384, 0, 1e-3)
     t, l = gpu_tree(vg, x, 64)

@@ -33,6 +33,7 @@ struct TreeDev {
     DevBuf<int32_t> int_ids;         // internal cells, level by level
     DevBuf<int32_t> leaf_ids;        // non-empty leaves
     DevBuf<uint8_t> flags;           // scratch for compaction
+    DevBuf<uint32_t> lv;             // level table of a device-driven build
 
     // interaction lists (reference order, device ids)
     DevBuf<int32_t> nb, ni, ap;              // 27, 27, 189 slots per cell

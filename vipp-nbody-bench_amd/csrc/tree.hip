@@ -281,10 +281,182 @@ __global__ __launch_bounds__(256) void k_leaf_sort(const int32_t *__restrict__ l
     }
 }
 
+// ---- device-driven level loop: the level sizes stay on the device -------
+// lv[d]: first cell of level d; lv[LV_I + d]: first entry of level d's
+// internal cells in int_ids; LV_OVF: a capacity or grid bound was exceeded
+// (bit 0) or the tree is deeper than the sorted digits (bit 1), and the host
+// then builds again level by level; LV_NLEAF: non-empty leaves.
+constexpr int LV_I = MAXD + 3, LV_OVF = 2 * (MAXD + 3), LV_NLEAF = LV_OVF + 1, LV_WORDS = LV_OVF + 2;
+
+__global__ void k_lv_init(const uint32_t *__restrict__ nint0, uint32_t *__restrict__ lv) {
+    for (int i = threadIdx.x; i < LV_WORDS; i += blockDim.x) lv[i] = 0;
+    if (threadIdx.x == 0) {
+        lv[1] = 1;
+        lv[LV_I + 1] = *nint0;
+    }
+}
+
+// k_expand with the internal cells of level d and the first new cell read from lv
+__global__ __launch_bounds__(256) void k_expand_lv(int d, uint32_t *lv, const int32_t *__restrict__ int_ids, int64_t cap_cells,
+                                                   const uint64_t *__restrict__ key, uint32_t thr, uint32_t *start, uint32_t *count, int32_t *child,
+                                                   int32_t *parent, uint8_t *octant, int8_t *level, double *bmin, double *bmax, int32_t *grid,
+                                                   int32_t *anc, uint8_t *flag) {
+    const uint32_t i0 = lv[LV_I + d], nint = lv[LV_I + d + 1] - i0;
+    const int64_t child_off = lv[d + 1];
+    if (lv[LV_OVF]) return;
+    if (child_off + 8ll * nint > cap_cells || 8ll * nint > static_cast<int64_t>(gridDim.x) * 256) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) lv[LV_OVF] = 1;  // every block sees the same sizes and returns
+        return;
+    }
+    const int64_t t = blockIdx.x * 256ll + threadIdx.x;
+    if (t >= 8ll * nint) return;
+    const int k = static_cast<int>(t >> 3);
+    const unsigned q = static_cast<unsigned>(t & 7);
+    const int32_t P = int_ids[i0 + k];
+    const uint32_t s = start[P], e = s + count[P];
+    const int shift = 3 * (MAXD - 1 - d);
+    const uint32_t lo = digit_lower_bound(key, s, e, shift, q);
+    const uint32_t hi = q == 7 ? e : digit_lower_bound(key, lo, e, shift, q + 1);
+    const int64_t c = child_off + 8ll * k + q;
+    start[c] = lo;
+    count[c] = hi - lo;
+    child[c] = -1;
+    parent[c] = P;
+    octant[c] = static_cast<uint8_t>(q);
+    level[c] = static_cast<int8_t>(d + 1);
+    for (int a = 0; a < 3; ++a) {
+        const double mn = bmin[3 * P + a], mx = bmax[3 * P + a];
+        const double mid = (mx + mn) / 2;
+        const bool low = (q >> a) & 1u;
+        bmin[3 * c + a] = low ? mn : mid;
+        bmax[3 * c + a] = low ? mid : mx;
+        grid[3 * c + a] = 2 * grid[3 * P + a] + (low ? 0 : 1);
+    }
+    for (int l = 0; l <= MAXD; ++l) anc[c * (MAXD + 1) + l] = l <= d ? anc[static_cast<int64_t>(P) * (MAXD + 1) + l] : (l == d + 1 ? static_cast<int32_t>(c) : -1);
+    flag[8ll * k + q] = (hi - lo) > thr ? 1 : 0;
+    if (q == 0) child[P] = static_cast<int32_t>(child_off + 8ll * k);
+}
+
+// What a device-driven compaction works on: the children of level d's
+// internal cells (d >= 0: flags at cell lv[d + 1] on, entries appended to
+// int_ids at lv[LV_I + d + 1]), or all cells below lv[-d - 1] (d < 0: the
+// leaves, entries from 0).
+struct LvCmp {
+    int64_t nflag, base, out0;
+};
+__device__ inline LvCmp lv_cmp(const uint32_t *lv, int d) {
+    if (lv[LV_OVF]) return {0, 0, 0};
+    if (d >= 0) return {8ll * (lv[LV_I + d + 1] - lv[LV_I + d]), lv[d + 1], lv[LV_I + d + 1]};
+    return {lv[-d - 1], 0, 0};
+}
+
+__global__ __launch_bounds__(1024) void k_flag_counts_lv(const uint8_t *__restrict__ flag, const uint32_t *__restrict__ lv, int d,
+                                                         uint32_t *__restrict__ bcnt) {
+    __shared__ uint32_t wsum[16];
+    const LvCmp a = lv_cmp(lv, d);
+    const int64_t i = blockIdx.x * 1024ll + threadIdx.x;
+    const uint64_t bal = __ballot(i < a.nflag && flag[i]);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = __popcll(bal);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < 16; ++w) t += wsum[w];
+        bcnt[blockIdx.x] = t;
+    }
+}
+
+// exclusive scan of the block counts in one block; the total goes into lv
+__global__ __launch_bounds__(1024) void k_count_scan_lv(uint32_t *__restrict__ bcnt, int nb, uint32_t *lv, int d, int64_t cap_out, bool last) {
+    __shared__ uint32_t carry;
+    __shared__ uint32_t wsum[16];
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    const unsigned lane = threadIdx.x & 63;
+    for (int off = 0; off < nb; off += 1024) {
+        const int i = off + threadIdx.x;
+        const uint32_t v = i < nb ? bcnt[i] : 0u;
+        uint32_t x = v;
+        for (int s = 1; s < 64; s <<= 1) {
+            const uint32_t o = __shfl_up(x, s, 64);
+            if (lane >= static_cast<unsigned>(s)) x += o;
+        }
+        if (lane == 63) wsum[threadIdx.x >> 6] = x;
+        __syncthreads();
+        uint32_t wb = 0;
+        for (unsigned w = 0; w < (threadIdx.x >> 6); ++w) wb += wsum[w];
+        const uint32_t c0 = carry;
+        if (i < nb) bcnt[i] = c0 + wb + x - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry = c0 + wb + x;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const uint32_t tot = carry;
+        if (d >= 0) {
+            const uint32_t nint = lv[LV_I + d + 1] - lv[LV_I + d];
+            lv[LV_I + d + 2] = lv[LV_I + d + 1] + tot;
+            lv[d + 2] = lv[d + 1] + 8u * nint;
+            if (lv[LV_I + d + 2] > cap_out) lv[LV_OVF] |= 1u;
+            if (last && tot > 0) lv[LV_OVF] |= 2u;  // cells of the last sorted level split further
+        } else {
+            lv[LV_NLEAF] = tot;
+        }
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_flag_write_lv(const uint8_t *__restrict__ flag, const uint32_t *__restrict__ lv, int d,
+                                                        const uint32_t *__restrict__ boff, int32_t *__restrict__ out, int64_t cap_out) {
+    __shared__ uint32_t wsum[16];
+    const LvCmp a = lv_cmp(lv, d);
+    const int64_t i = blockIdx.x * 1024ll + threadIdx.x;
+    const unsigned lane = threadIdx.x & 63;
+    const bool f = i < a.nflag && flag[i];
+    const uint64_t bal = __ballot(f);
+    if (lane == 0) wsum[threadIdx.x >> 6] = __popcll(bal);
+    __syncthreads();
+    uint32_t wb = boff[blockIdx.x];
+    for (unsigned w = 0; w < (threadIdx.x >> 6); ++w) wb += wsum[w];
+    const int64_t pos = a.out0 + wb + __popcll(bal & ((1ull << lane) - 1));
+    if (f && pos < cap_out) out[pos] = static_cast<int32_t>(a.base + i);
+}
+
+__global__ __launch_bounds__(256) void k_leaf_flags_lv(const int32_t *__restrict__ child, const uint32_t *__restrict__ count, const uint32_t *lv,
+                                                       int last, int64_t cap, uint8_t *__restrict__ flag) {
+    const int64_t c = blockIdx.x * 256ll + threadIdx.x;
+    if (c >= cap) return;
+    const int64_t nc = lv[last];
+    flag[c] = (c < nc && child[c] < 0 && count[c] > 0) ? 1 : 0;
+}
+
+// k_leaf_sort over the lv[LV_NLEAF] leaves, a leaf per block at a time
+__global__ __launch_bounds__(256) void k_leaf_sort_lv(const int32_t *__restrict__ leaves, const uint32_t *lv, const uint32_t *__restrict__ start,
+                                                      const uint32_t *__restrict__ count, uint32_t thr, const uint32_t *__restrict__ in,
+                                                      uint32_t *__restrict__ out) {
+    extern __shared__ uint32_t sv[];  // thr words
+    if (lv[LV_OVF]) return;
+    const uint32_t nl = lv[LV_NLEAF];
+    for (uint32_t j = blockIdx.x; j < nl; j += gridDim.x) {
+        const int32_t c = leaves[j];
+        const uint32_t s = start[c], n = count[c];
+        if (n > thr) continue;  // not a leaf of a complete build (same n for the whole block)
+        __syncthreads();  // the previous leaf's ranks are read
+        for (uint32_t i = threadIdx.x; i < n; i += 256) sv[i] = in[s + i];
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < n; i += 256) {
+            const uint32_t v = sv[i];
+            uint32_t r = 0;
+            for (uint32_t k = 0; k < n; ++k) r += sv[k] < v;
+            out[s + r] = v;
+        }
+    }
+}
+
+// lv (optional): skipped when a device-driven build did not complete
 __global__ __launch_bounds__(256) void k_gather(const uint32_t *__restrict__ perm, int64_t n, const double *__restrict__ x, const double *__restrict__ m,
-                                                double mass, const double *__restrict__ box, double *__restrict__ xs64, float *__restrict__ xs32) {
+                                                double mass, const double *__restrict__ box, double *__restrict__ xs64, float *__restrict__ xs32,
+                                                const uint32_t *lv) {
     const int64_t k = blockIdx.x * 256ll + threadIdx.x;
-    if (k >= n) return;
+    if (k >= n || (lv && lv[LV_OVF])) return;
     const uint32_t i = perm[k];
     const double px = x[3ll * i], py = x[3ll * i + 1], pz = x[3ll * i + 2];
     const double mi = m ? m[i] : mass;
@@ -495,6 +667,102 @@ static void compact_flags(Ctx &c, const uint8_t *flag, int64_t n, int64_t base, 
     VG_LAUNCH_CHECK();
 }
 
+static void ensure_cells(TreeDev &t, int64_t ncell, bool keep, hipStream_t st) {
+    t.start.ensure(ncell, keep, st);
+    t.count.ensure(ncell, keep, st);
+    t.child.ensure(ncell, keep, st);
+    t.parent.ensure(ncell, keep, st);
+    t.octant.ensure(ncell, keep, st);
+    t.level.ensure(ncell, keep, st);
+    t.bmin.ensure(3 * ncell, keep, st);
+    t.bmax.ensure(3 * ncell, keep, st);
+    t.grid.ensure(3 * ncell, keep, st);
+    t.anc.ensure((MAXD + 1) * ncell, keep, st);
+}
+
+// Steps 2-4 of tree_build with the level sizes kept on the device: every
+// level of the sorted digits is launched at once, on grids sized by bounds
+// (level d holds at most min(8^d, n / (thr + 1)) internal cells, whose points
+// are disjoint sets of more than thr each), and the host reads the level
+// table once. Returns 1 when built, 0 when the cell capacity (twice the
+// previous tree's cells) was too small, -1 when the tree is deeper than the
+// sorted digits; the caller then builds level by level.
+static int tree_build_lv(Ctx &c, int SH, uint64_t *&k0, uint32_t *&v0, uint64_t *&k1, uint32_t *&v1) {
+    hipStream_t st = c.stream;
+    TreeDev &t = c.t;
+    const int64_t n = c.n;
+    k_codes<<<ceil_div(n, 256), 256, 0, st>>>(c.x, n, c.box, k0, v0);
+    VG_LAUNCH_CHECK();
+    radix_sort_pairs(st, k0, v0, k1, v1, n, 3 * (MAXD - SH), 3 * MAXD, c.rs_hist);
+    const int64_t cap = std::max<int64_t>(1024, 2 * t.ncells);
+    ensure_cells(t, cap, false, st);
+    t.int_ids.ensure(cap, false, st);
+    t.leaf_ids.ensure(cap, false, st);
+    const int64_t per = n / (static_cast<int64_t>(c.thr) + 1);
+    std::vector<int64_t> nflag(SH);
+    int64_t fmax = cap;
+    for (int d = 0; d < SH; ++d) {
+        const int64_t b = d < 7 ? std::min<int64_t>(per, 1ll << (3 * d)) : per;
+        nflag[d] = std::max<int64_t>(1, 8 * b);
+        fmax = std::max(fmax, nflag[d]);
+    }
+    t.flags.ensure(ceil_div(fmax, 1024) * 1024, false, st);
+    c.cmp_cnt.ensure(ceil_div(fmax, 1024));
+    t.lv.ensure(LV_WORDS);
+    k_root<<<1, 64, 0, st>>>(c.box, n, c.thr, t.start, t.count, t.child, t.parent, t.octant, t.level, t.bmin, t.bmax, t.grid, t.anc, t.int_ids,
+                            c.scal.p);
+    k_lv_init<<<1, 64, 0, st>>>(c.scal.p, t.lv);
+    for (int d = 0; d < SH; ++d) {
+        const unsigned gc = static_cast<unsigned>(ceil_div(nflag[d], 1024));
+        k_expand_lv<<<ceil_div(nflag[d], 256), 256, 0, st>>>(d, t.lv, t.int_ids, cap, k0, c.thr, t.start, t.count, t.child, t.parent, t.octant,
+                                                              t.level, t.bmin, t.bmax, t.grid, t.anc, t.flags);
+        k_flag_counts_lv<<<gc, 1024, 0, st>>>(t.flags, t.lv, d, c.cmp_cnt);
+        k_count_scan_lv<<<1, 1024, 0, st>>>(c.cmp_cnt, static_cast<int>(gc), t.lv, d, cap, d == SH - 1);
+        k_flag_write_lv<<<gc, 1024, 0, st>>>(t.flags, t.lv, d, c.cmp_cnt, t.int_ids, cap);
+    }
+    VG_LAUNCH_CHECK();
+    // non-empty leaves among the lv[SH + 1] cells
+    const unsigned gl = static_cast<unsigned>(ceil_div(cap, 1024));
+    k_leaf_flags_lv<<<ceil_div(cap, 256), 256, 0, st>>>(t.child, t.count, t.lv, SH + 1, cap, t.flags);
+    k_flag_counts_lv<<<gl, 1024, 0, st>>>(t.flags, t.lv, -(SH + 2), c.cmp_cnt);
+    k_count_scan_lv<<<1, 1024, 0, st>>>(c.cmp_cnt, static_cast<int>(gl), t.lv, -(SH + 2), cap, false);
+    k_flag_write_lv<<<gl, 1024, 0, st>>>(t.flags, t.lv, -(SH + 2), c.cmp_cnt, t.leaf_ids, cap);
+    c.perm.ensure(std::max<int64_t>(1, n));
+    // a block per leaf of the previous tree (and a quarter more): the leaf count changes little
+    const int64_t gls = std::min<int64_t>(cap, std::max<int64_t>(1024, t.nleaves + t.nleaves / 4));
+    k_leaf_sort_lv<<<static_cast<unsigned>(gls), 256, c.thr * sizeof(uint32_t), st>>>(t.leaf_ids, t.lv, t.start, t.count, c.thr,
+                                                                                                              v0, c.perm);
+    c.xs64.ensure(4 * std::max<int64_t>(1, n));
+    c.xs32.ensure(4 * std::max<int64_t>(1, n));
+    k_gather<<<ceil_div(n, 256), 256, 0, st>>>(c.perm, n, c.x, c.has_masses ? c.m.p : nullptr, c.mass, c.box, c.xs64, c.xs32, t.lv);
+    VG_LAUNCH_CHECK();
+    double *hbox = static_cast<double *>(c.pinned);                                         // 8 doubles
+    uint32_t *hlv = reinterpret_cast<uint32_t *>(static_cast<char *>(c.pinned) + 512);  // LV_WORDS pinned host words
+    static_assert(512 + 4 * LV_WORDS <= Ctx::PINNED_BYTES, "pinned words");
+    VG_HIP(hipMemcpyAsync(hbox, c.box, 7 * sizeof(double), hipMemcpyDeviceToHost, st));
+    VG_HIP(hipMemcpyAsync(hlv, t.lv, LV_WORDS * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    VG_HIP(hipStreamSynchronize(st));
+    if (hbox[6] != 0) throw Error(2, "non-finite particle coordinates");
+    if (hlv[LV_OVF] & 1u) return 0;
+    if (hlv[LV_OVF]) return -1;
+    std::vector<int64_t> lvl_off{0, 1}, int_off{0, hlv[LV_I + 1]};
+    int64_t nint = hlv[LV_I + 1];
+    for (int d = 0; nint > 0; ++d) {
+        if (d >= SH) return -1;  // level d + 1 digits were not sorted
+        lvl_off.push_back(hlv[d + 2]);
+        int_off.push_back(hlv[LV_I + d + 2]);
+        nint = static_cast<int64_t>(hlv[LV_I + d + 2]) - hlv[LV_I + d + 1];
+    }
+    for (int a = 0; a < 6; ++a) t.box[a] = hbox[a];
+    t.lvl_off.swap(lvl_off);
+    t.int_off.swap(int_off);
+    t.depth = static_cast<int>(t.lvl_off.size()) - 2;
+    t.ncells = t.lvl_off.back();
+    t.nleaves = hlv[LV_NLEAF];
+    t.ref_valid = false;
+    return 1;
+}
+
 void tree_build(Ctx &c) {
     hipStream_t st = c.stream;
     TreeDev &t = c.t;
@@ -522,7 +790,13 @@ void tree_build(Ctx &c) {
     }
     uint64_t *k0 = c.key[0].p, *k1 = c.key[1].p;
     uint32_t *v0 = c.val[0].p, *v1 = c.val[1].p;
-    for (int attempt = 0; attempt < 2; ++attempt) {
+    int first = 0;
+    if (t.depth > 0 && n > 0) {  // a previous tree sizes the capacity and the digits sorted
+        const int r = tree_build_lv(c, SHALLOW, k0, v0, k1, v1);
+        if (r > 0) return;
+        if (r < 0) first = 1;
+    }
+    for (int attempt = first; attempt < 2; ++attempt) {
     const int low_bit = attempt == 0 ? 3 * (MAXD - SHALLOW) : 0;
     if (n) {
         k_codes<<<ceil_div(n, 256), 256, 0, st>>>(c.x, n, c.box, k0, v0);
@@ -533,19 +807,7 @@ void tree_build(Ctx &c) {
     t.lvl_off.assign(1, 0);
     t.int_off.assign(1, 0);
     int64_t cap = 1024;
-    auto ensure_cells = [&](int64_t ncell, bool keep) {
-        t.start.ensure(ncell, keep, st);
-        t.count.ensure(ncell, keep, st);
-        t.child.ensure(ncell, keep, st);
-        t.parent.ensure(ncell, keep, st);
-        t.octant.ensure(ncell, keep, st);
-        t.level.ensure(ncell, keep, st);
-        t.bmin.ensure(3 * ncell, keep, st);
-        t.bmax.ensure(3 * ncell, keep, st);
-        t.grid.ensure(3 * ncell, keep, st);
-        t.anc.ensure((MAXD + 1) * ncell, keep, st);
-    };
-    ensure_cells(cap, false);
+    ensure_cells(t, cap, false, st);
     t.int_ids.ensure(1024, false, st);
     k_root<<<1, 64, 0, st>>>(c.box, n, c.thr, t.start, t.count, t.child, t.parent, t.octant, t.level, t.bmin, t.bmax, t.grid, t.anc, t.int_ids,
                             c.scal.p);
@@ -572,7 +834,7 @@ void tree_build(Ctx &c) {
         const int64_t nnew = 8 * nint;
         if (ncells + nnew > cap) {
             while (cap < ncells + nnew) cap *= 2;
-            ensure_cells(cap, true);
+            ensure_cells(t, cap, true, st);
         }
         c.t.flags.ensure(nnew, false, st);
         t.int_ids.ensure(t.int_off.back() + nnew, true, st);
@@ -608,7 +870,7 @@ void tree_build(Ctx &c) {
     c.xs64.ensure(4 * std::max<int64_t>(1, n));
     c.xs32.ensure(4 * std::max<int64_t>(1, n));
     if (n) {
-        k_gather<<<ceil_div(n, 256), 256, 0, st>>>(c.perm, n, c.x, c.has_masses ? c.m.p : nullptr, c.mass, c.box, c.xs64, c.xs32);
+        k_gather<<<ceil_div(n, 256), 256, 0, st>>>(c.perm, n, c.x, c.has_masses ? c.m.p : nullptr, c.mass, c.box, c.xs64, c.xs32, nullptr);
         VG_LAUNCH_CHECK();
     }
     t.ref_valid = false;
