@@ -81,15 +81,17 @@ __global__ __launch_bounds__(256) void k_aabb_final(const double *__restrict__ p
     if (threadIdx.x == 0) box[6] = s[6][0];
 }
 
-// 63-bit path code: level-1 octant in bits 62..60, level-21 octant in bits 2..0.
-__global__ __launch_bounds__(256) void k_codes(const double *__restrict__ x, int64_t n, const double *__restrict__ box, uint64_t *__restrict__ key,
-                                               uint32_t *__restrict__ val) {
+// 63-bit path code: level-1 octant in bits 62..60, level-21 octant in bits 2..0;
+// the octants of levels 1..levels only (the bits of deeper levels stay 0: the
+// tree reads no digit below the levels it sorts).
+__global__ __launch_bounds__(256) void k_codes(const double *__restrict__ x, int64_t n, const double *__restrict__ box, int levels,
+                                               uint64_t *__restrict__ key, uint32_t *__restrict__ val) {
     const int64_t i = blockIdx.x * 256ll + threadIdx.x;
     if (i >= n) return;
     double lo[3] = {box[0], box[1], box[2]}, hi[3] = {box[3], box[4], box[5]};
     const double p[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
     uint64_t code = 0;
-    for (int l = 0; l < MAXD; ++l) {
+    for (int l = 0; l < levels; ++l) {
         unsigned o = 0;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
@@ -103,7 +105,7 @@ __global__ __launch_bounds__(256) void k_codes(const double *__restrict__ x, int
         }
         code = (code << 3) | o;
     }
-    key[i] = code;
+    key[i] = code << (3 * (MAXD - levels));
     val[i] = static_cast<uint32_t>(i);
 }
 
@@ -691,7 +693,7 @@ static int tree_build_lv(Ctx &c, int SH, uint64_t *&k0, uint32_t *&v0, uint64_t 
     hipStream_t st = c.stream;
     TreeDev &t = c.t;
     const int64_t n = c.n;
-    k_codes<<<ceil_div(n, 256), 256, 0, st>>>(c.x, n, c.box, k0, v0);
+    k_codes<<<ceil_div(n, 256), 256, 0, st>>>(c.x, n, c.box, SH, k0, v0);
     VG_LAUNCH_CHECK();
     radix_sort_pairs(st, k0, v0, k1, v1, n, 3 * (MAXD - SH), 3 * MAXD, c.rs_hist);
     const int64_t cap = std::max<int64_t>(1024, 2 * t.ncells);
@@ -799,7 +801,7 @@ void tree_build(Ctx &c) {
     for (int attempt = first; attempt < 2; ++attempt) {
     const int low_bit = attempt == 0 ? 3 * (MAXD - SHALLOW) : 0;
     if (n) {
-        k_codes<<<ceil_div(n, 256), 256, 0, st>>>(c.x, n, c.box, k0, v0);
+        k_codes<<<ceil_div(n, 256), 256, 0, st>>>(c.x, n, c.box, attempt == 0 ? SHALLOW : MAXD, k0, v0);
         VG_LAUNCH_CHECK();
         radix_sort_pairs(st, k0, v0, k1, v1, n, low_bit, 3 * MAXD, c.rs_hist);
     }
