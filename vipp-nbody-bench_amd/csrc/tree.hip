@@ -430,26 +430,39 @@ __global__ __launch_bounds__(256) void k_leaf_flags_lv(const int32_t *__restrict
     flag[c] = (c < nc && child[c] < 0 && count[c] > 0) ? 1 : 0;
 }
 
-// k_leaf_sort over the lv[LV_NLEAF] leaves, a leaf per block at a time
-__global__ __launch_bounds__(256) void k_leaf_sort_lv(const int32_t *__restrict__ leaves, const uint32_t *lv, const uint32_t *__restrict__ start,
-                                                      const uint32_t *__restrict__ count, uint32_t thr, const uint32_t *__restrict__ in,
-                                                      uint32_t *__restrict__ out) {
-    extern __shared__ uint32_t sv[];  // thr words
-    if (lv[LV_OVF]) return;
-    const uint32_t nl = lv[LV_NLEAF];
-    for (uint32_t j = blockIdx.x; j < nl; j += gridDim.x) {
+// k_leaf_sort with a wavefront per leaf (LS_WAVES leaves per block, LDS of
+// LS_WAVES thr words): the leaves hold about s points, so a block per leaf
+// would leave most of its threads idle. nl leaves, or with lv the
+// lv[LV_NLEAF] leaves of a device-driven build (none when it did not complete).
+constexpr int LS_WAVES = 4;
+constexpr uint32_t LS_MAXTHR = 4096;  // LDS bound of the wavefront form
+__global__ __launch_bounds__(64 * LS_WAVES) void k_leaf_sort_w(const int32_t *__restrict__ leaves, uint32_t nl, const uint32_t *lv,
+                                                               const uint32_t *__restrict__ start, const uint32_t *__restrict__ count, uint32_t thr,
+                                                               const uint32_t *__restrict__ in, uint32_t *__restrict__ out) {
+    extern __shared__ uint32_t sv[];
+    if (lv) {
+        if (lv[LV_OVF]) return;
+        nl = lv[LV_NLEAF];
+    }
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint32_t *sw = sv + w * thr;
+    for (uint32_t j = blockIdx.x * LS_WAVES + w; j < nl; j += gridDim.x * LS_WAVES) {
         const int32_t c = leaves[j];
         const uint32_t s = start[c], n = count[c];
-        if (n > thr) continue;  // not a leaf of a complete build (same n for the whole block)
-        __syncthreads();  // the previous leaf's ranks are read
-        for (uint32_t i = threadIdx.x; i < n; i += 256) sv[i] = in[s + i];
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < n; i += 256) {
-            const uint32_t v = sv[i];
+        if (n > thr) continue;  // not a leaf of a complete build (same n for the whole wavefront)
+        for (uint32_t i = lane; i < n; i += 64) sw[i] = in[s + i];
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (uint32_t i = lane; i < n; i += 64) {
+            const uint32_t v = sw[i];
             uint32_t r = 0;
-            for (uint32_t k = 0; k < n; ++k) r += sv[k] < v;
+            for (uint32_t k = 0; k < n; ++k) r += sw[k] < v;
             out[s + r] = v;
         }
+        __builtin_amdgcn_wave_barrier();  // every lane has read sw before the next leaf overwrites it
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
@@ -730,10 +743,11 @@ static int tree_build_lv(Ctx &c, int SH, uint64_t *&k0, uint32_t *&v0, uint64_t 
     k_count_scan_lv<<<1, 1024, 0, st>>>(c.cmp_cnt, static_cast<int>(gl), t.lv, -(SH + 2), cap, false);
     k_flag_write_lv<<<gl, 1024, 0, st>>>(t.flags, t.lv, -(SH + 2), c.cmp_cnt, t.leaf_ids, cap);
     c.perm.ensure(std::max<int64_t>(1, n));
-    // a block per leaf of the previous tree (and a quarter more): the leaf count changes little
-    const int64_t gls = std::min<int64_t>(cap, std::max<int64_t>(1024, t.nleaves + t.nleaves / 4));
-    k_leaf_sort_lv<<<static_cast<unsigned>(gls), 256, c.thr * sizeof(uint32_t), st>>>(t.leaf_ids, t.lv, t.start, t.count, c.thr,
-                                                                                                              v0, c.perm);
+    // a wavefront per leaf of the previous tree and a quarter more (the leaf
+    // count changes little; the blocks loop over any excess)
+    const int64_t gls = ceil_div(std::min<int64_t>(cap, std::max<int64_t>(1024, t.nleaves + t.nleaves / 4)), LS_WAVES);
+    k_leaf_sort_w<<<static_cast<unsigned>(gls), 64 * LS_WAVES, LS_WAVES * c.thr * sizeof(uint32_t), st>>>(t.leaf_ids, 0, t.lv, t.start, t.count,
+                                                                                                          c.thr, v0, c.perm);
     c.xs64.ensure(4 * std::max<int64_t>(1, n));
     c.xs32.ensure(4 * std::max<int64_t>(1, n));
     k_gather<<<ceil_div(n, 256), 256, 0, st>>>(c.perm, n, c.x, c.has_masses ? c.m.p : nullptr, c.mass, c.box, c.xs64, c.xs32, t.lv);
@@ -793,7 +807,7 @@ void tree_build(Ctx &c) {
     uint64_t *k0 = c.key[0].p, *k1 = c.key[1].p;
     uint32_t *v0 = c.val[0].p, *v1 = c.val[1].p;
     int first = 0;
-    if (t.depth > 0 && n > 0) {  // a previous tree sizes the capacity and the digits sorted
+    if (t.depth > 0 && n > 0 && c.thr <= LS_MAXTHR) {  // a previous tree sizes the capacity and the digits sorted
         const int r = tree_build_lv(c, SHALLOW, k0, v0, k1, v1);
         if (r > 0) return;
         if (r < 0) first = 1;
@@ -866,7 +880,11 @@ void tree_build(Ctx &c) {
     t.nleaves = hbuf[0];
     c.perm.ensure(std::max<int64_t>(1, n));
     if (t.nleaves) {
-        k_leaf_sort<<<static_cast<unsigned>(t.nleaves), 256, c.thr * sizeof(uint32_t), st>>>(t.leaf_ids, t.start, t.count, v0, c.perm);
+        if (c.thr <= LS_MAXTHR)
+            k_leaf_sort_w<<<static_cast<unsigned>(ceil_div(t.nleaves, LS_WAVES)), 64 * LS_WAVES, LS_WAVES * c.thr * sizeof(uint32_t), st>>>(
+                t.leaf_ids, static_cast<uint32_t>(t.nleaves), nullptr, t.start, t.count, c.thr, v0, c.perm);
+        else
+            k_leaf_sort<<<static_cast<unsigned>(t.nleaves), 256, c.thr * sizeof(uint32_t), st>>>(t.leaf_ids, t.start, t.count, v0, c.perm);
         VG_LAUNCH_CHECK();
     }
     c.xs64.ensure(4 * std::max<int64_t>(1, n));
