@@ -75,11 +75,12 @@ void join_p2p(Ctx &c) {
     c.p2p_pending = false;
 }
 
-// Untimed evaluations launch P2P (VALU-bound, about half the evaluation) on a
-// second stream once the tree and lists exist, so that the small launches of
-// the operators and the upward pass, and M2L (MFMA-bound), run beside it; L2P
-// waits for it. Timed evaluations keep one stream, so that each phase's events
-// bracket its own kernels only.
+// Untimed split-mode evaluations launch P2P (VALU-bound, about half the
+// evaluation) on a second stream once the tree and lists exist, so that the
+// small launches of the operators and the upward pass, and M2L (MFMA-bound),
+// run beside it; L2P waits for it. The FP32 and FP64 modes, whose M2L is
+// VALU-bound too, and timed evaluations keep one stream (the events of a
+// phase then bracket its own kernels only).
 static void evaluate_overlapped(Ctx &c, bool potential) {
     VG_HIP(hipEventRecord(c.ev_fork, c.stream));
     VG_HIP(hipStreamWaitEvent(c.stream2, c.ev_fork, 0));
@@ -104,7 +105,7 @@ static void evaluate_overlapped(Ctx &c, bool potential) {
 void evaluate(Ctx &c, bool potential) {
     join_p2p(c);  // a P2P left pending by an evaluation that threw must end before the tree changes
     bool used[PH_COUNT] = {};
-    if (!c.timing && c.stream2) {
+    if (!c.timing && c.stream2 && c.prec == 2) {
         tree_build(c);
         lists_build(c);
         evaluate_overlapped(c, potential);
