@@ -75,27 +75,39 @@ void join_p2p(Ctx &c) {
     c.p2p_pending = false;
 }
 
-// Untimed split-mode evaluations launch P2P (VALU-bound, about half the
-// evaluation) on a second stream once the tree and lists exist, so that the
-// small launches of the operators and the upward pass, and M2L (MFMA-bound),
-// run beside it; L2P waits for it. The FP32 and FP64 modes, whose M2L is
-// VALU-bound too, and timed evaluations keep one stream (the events of a
-// phase then bracket its own kernels only).
+// Untimed split-mode evaluations run the lists and P2P (VALU-bound, about
+// half the evaluation) on a second stream once the tree exists, so that the
+// operators and the upward pass (which need only the tree), and M2L
+// (MFMA-bound, after the lists), run beside them; L2P waits for P2P. The FP32
+// and FP64 modes, whose M2L is VALU-bound too, and timed evaluations keep one
+// stream (the events of a phase then bracket its own kernels only).
 static void evaluate_overlapped(Ctx &c, bool potential) {
     VG_HIP(hipEventRecord(c.ev_fork, c.stream));
     VG_HIP(hipStreamWaitEvent(c.stream2, c.ev_fork, 0));
-    operators_build(c);
-    fmm_upward(c);  // enqueued first: the GPU has work while the host waits for the P2P plan
-    {
-        struct Swap {
-            Ctx &c;
-            explicit Swap(Ctx &c_) : c(c_) { std::swap(c.stream, c.stream2); }
-            ~Swap() { std::swap(c.stream, c.stream2); }
-        } on_stream2(c);
-        fmm_p2p(c, potential);
+    struct Swap {
+        Ctx &c;
+        explicit Swap(Ctx &c_) : c(c_) { std::swap(c.stream, c.stream2); }
+        ~Swap() { std::swap(c.stream, c.stream2); }
+    };
+    try {
+        {
+            Swap on_stream2(c);
+            lists_build(c);
+        }
+        VG_HIP(hipEventRecord(c.ev_lists, c.stream2));
+        operators_build(c);
+        fmm_upward(c);  // enqueued first: the GPU has work while the host waits for the P2P plan
+        {
+            Swap on_stream2(c);
+            fmm_p2p(c, potential);
+        }
+        VG_HIP(hipEventRecord(c.ev_join, c.stream2));
+    } catch (...) {
+        (void)hipStreamSynchronize(c.stream2);  // nothing of this evaluation stays in flight
+        throw;
     }
-    VG_HIP(hipEventRecord(c.ev_join, c.stream2));
     c.p2p_pending = true;
+    VG_HIP(hipStreamWaitEvent(c.stream, c.ev_lists, 0));  // M2L reads the parents' neighbour tables
     fmm_m2l(c);
     fmm_downward(c, potential);
     join_p2p(c);
@@ -107,7 +119,6 @@ void evaluate(Ctx &c, bool potential) {
     bool used[PH_COUNT] = {};
     if (!c.timing && c.stream2 && c.prec == 2) {
         tree_build(c);
-        lists_build(c);
         evaluate_overlapped(c, potential);
         c.evaluated = true;
         return;
@@ -378,6 +389,7 @@ int vipp_grav_create(const vipp_grav_config *cfg, vipp_grav_ctx **out) {
             VG_HIP(hipStreamCreateWithFlags(&c.stream2, hipStreamNonBlocking));
             VG_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
             VG_HIP(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
+            VG_HIP(hipEventCreateWithFlags(&c.ev_lists, hipEventDisableTiming));
             VG_HIP(hipHostMalloc(&c.pinned, vg::Ctx::PINNED_BYTES, hipHostMallocDefault));
             for (auto &e : c.ev) VG_HIP(hipEventCreate(&e));
             vg::fmm_constants_init();
@@ -407,6 +419,7 @@ int vipp_grav_destroy(vipp_grav_ctx *ctx) {
         if (e) (void)hipEventDestroy(e);
     if (ctx->c.ev_fork) (void)hipEventDestroy(ctx->c.ev_fork);
     if (ctx->c.ev_join) (void)hipEventDestroy(ctx->c.ev_join);
+    if (ctx->c.ev_lists) (void)hipEventDestroy(ctx->c.ev_lists);
     if (ctx->c.stream) (void)hipStreamDestroy(ctx->c.stream);
     if (ctx->c.stream2) (void)hipStreamDestroy(ctx->c.stream2);
     if (ctx->c.pinned) (void)hipHostFree(ctx->c.pinned);

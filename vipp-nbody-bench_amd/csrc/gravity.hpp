@@ -94,10 +94,11 @@ struct Ctx {
     std::string err;
 
     hipStream_t stream = nullptr;
-    // P2P runs on stream2 beside the upward pass and M2L of the same
-    // evaluation (they share no buffer); L2P waits for it (join_p2p)
+    // split mode: the lists and P2P run on stream2 beside the upward pass and
+    // M2L of the same evaluation (they share no buffer); M2L waits for the
+    // lists (ev_lists), L2P for P2P (join_p2p)
     hipStream_t stream2 = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_lists = nullptr;
     bool p2p_pending = false;
     // pinned host words for the scalars read back during an evaluation (tree
     // level sizes, root box, P2P plan): a pinned copy returns sooner
