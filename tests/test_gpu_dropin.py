@@ -15,6 +15,14 @@ source cell by value (fmm.hpp:281) while other threads of the flat M2L loop
 CPU alone, 2 of 12 runs at 16 threads ended in a segfault or std::bad_alloc;
 16 of 16 single-threaded runs were clean).
 
+test_reference_different_precision: the same driver in its "precision" mode
+does what the reference's main.cpp:33-38 does when built with
+ENABLE_DIFFERENT_PRECISION: Benchmark::benchmark_different_precision
+(benchmark.hpp:31-87) runs GpuSolver over one of main.cpp's type sets (double;
+float with a double time type) against GpuSolver<float, ...> as its reference,
+and must print the pass verdict of the same rule.  Both solvers run on the
+device; the reference's CPU solver is not involved.
+
 test_cpp_wrapper: tests/cpp/solver_gpu_test.cpp (built by build.py): the
 wrapper's three calls give bit-identical state to the C ABI driven directly,
 contexts are reused, errors surface as std::runtime_error."""
@@ -41,6 +49,18 @@ def test_reference_benchmark_hook(mode):
     assert r.returncode == 0, r.stderr
     m = re.search(r"α:(\S+) c:(\S+)", r.stdout)  # benchmark.hpp:160
     assert m, r.stdout
+    assert "solver has passed the benchmark test" in r.stdout, r.stdout
+
+
+@pytest.mark.parametrize("types,ref", [("double", "f32"), ("double", "split"), ("time_double", "split")])
+def test_reference_different_precision(types, ref):
+    if not os.path.exists(DROPIN):
+        pytest.skip("oracle/_ref/dropin is built only where the reference checkout exists")
+    r = subprocess.run([DROPIN, "4096", "10", "precision", types, ref], capture_output=True, text=True, timeout=240)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, r.stderr
+    m = re.search(r"α:(\S+) c:(\S+)", r.stdout)  # benchmark.hpp:75
+    assert m and all(v not in ("nan", "-nan", "inf", "-inf") for v in m.groups()), r.stdout
     assert "solver has passed the benchmark test" in r.stdout, r.stdout
 
 
