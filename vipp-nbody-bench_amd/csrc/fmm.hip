@@ -242,11 +242,22 @@ __global__ __launch_bounds__(256) void k_p2m(const int32_t *__restrict__ leaves,
         aim[k] = 0;
     }
     if (act) {
+        // the next particle of this lane group is loaded before the current
+        // one's harmonics are formed, so its latency hides behind them
+        const double2 *xs2 = reinterpret_cast<const double2 *>(xs64);
+        double2 nxy = make_double2(0, 0), nzm = make_double2(0, 0);
+        if (g < cnt) {
+            nxy = xs2[2 * static_cast<size_t>(s + g)];
+            nzm = xs2[2 * static_cast<size_t>(s + g) + 1];
+        }
         for (uint32_t j = g; j < cnt; j += G) {
-            const uint32_t k = s + j;
-            const R x = static_cast<R>((xs64[4 * k] - cx) * iw), y = static_cast<R>((xs64[4 * k + 1] - cy) * iw),
-                    z = static_cast<R>((xs64[4 * k + 2] - cz) * iw);
-            const R mi = static_cast<R>(xs64[4 * k + 3]);
+            const double2 cxy = nxy, czm = nzm;
+            if (j + G < cnt) {
+                nxy = xs2[2 * static_cast<size_t>(s + j + G)];
+                nzm = xs2[2 * static_cast<size_t>(s + j + G) + 1];
+            }
+            const R x = static_cast<R>((cxy.x - cx) * iw), y = static_cast<R>((cxy.y - cy) * iw), z = static_cast<R>((czm.x - cz) * iw);
+            const R mi = static_cast<R>(czm.y);
             const R r2 = x * x + y * y + z * z;
             // (x + i y)^m1 and ^m2 by repeated squaring: w^(2^b) once, a
             // multiply per set bit of each exponent (selects, no branches)
