@@ -203,12 +203,13 @@ __global__ __launch_bounds__(128) void k_ops(const double *__restrict__ harm, in
 // from; step s accumulates m_i conj(R) into accumulator s.  The groups are
 // then summed in fixed order through LDS by all 64 lanes.
 // ---------------------------------------------------------------------------
-template <class R>
+template <class R, int P>
 __global__ __launch_bounds__(256) void k_p2m(const int32_t *__restrict__ leaves, int nleaves, const int8_t *__restrict__ level,
                                              const uint32_t *__restrict__ start, const uint32_t *__restrict__ count, const double *__restrict__ bmin,
-                                             const double *__restrict__ bmax, const double *__restrict__ xs64, double wx0, int p, int K, int ldk,
+                                             const double *__restrict__ bmax, const double *__restrict__ xs64, double wx0, int K, int ldk,
                                              R *__restrict__ M) {
-    constexpr int NS = MAXP + 2;  // recurrence steps per lane
+    constexpr int p = P;       // the order is a template argument, so the recurrence unrolls fully
+    constexpr int NS = P + 2;  // recurrence steps per lane
     __shared__ R red[4][64][2 * NS];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int li = blockIdx.x * 4 + w;
@@ -727,9 +728,27 @@ void upward_t(Ctx &c) {
     TreeDev &t = c.t;
     Bufs<R>::ensure(c, t.ncells);
     const double wx0 = t.box[3] - t.box[0];
-    if (t.nleaves)
-        k_p2m<R><<<ceil_div(t.nleaves, 4), 256, 0, c.stream>>>(t.leaf_ids, static_cast<int>(t.nleaves), t.level, t.start, t.count, t.bmin, t.bmax,
-                                                               c.xs64, wx0, c.p, c.K, c.ldk, Bufs<R>::M(c));
+    if (t.nleaves) {
+#define VG_P2M(P_)                                                                                                                          \
+    k_p2m<R, P_><<<ceil_div(t.nleaves, 4), 256, 0, c.stream>>>(t.leaf_ids, static_cast<int>(t.nleaves), t.level, t.start, t.count, t.bmin, \
+                                                               t.bmax, c.xs64, wx0, c.K, c.ldk, Bufs<R>::M(c))
+        switch (c.p) {
+            case 1: VG_P2M(1); break;
+            case 2: VG_P2M(2); break;
+            case 3: VG_P2M(3); break;
+            case 4: VG_P2M(4); break;
+            case 5: VG_P2M(5); break;
+            case 6: VG_P2M(6); break;
+            case 7: VG_P2M(7); break;
+            case 8: VG_P2M(8); break;
+            case 9: VG_P2M(9); break;
+            case 10: VG_P2M(10); break;
+            case 11: VG_P2M(11); break;
+            case 12: VG_P2M(12); break;
+            default: throw Error(1, "P2M supports 1 <= p <= 12");
+        }
+#undef VG_P2M
+    }
     for (int d = t.depth; d >= 2; --d) {
         const int64_t a = t.int_off[d], b = t.int_off[d + 1];
         if (b > a) m2m_level<R>(c, t.int_ids.p + a, b - a);
