@@ -357,6 +357,27 @@ def test_coincident_points_depth_error(vg):
         assert e.value.code == vg.E_DEPTH
 
 
+def test_depth_error_then_recovery(vg, orc):
+    """A context whose previous tree exists builds the next from the sorted
+    digits of two more levels; coincident points then reach the 21-level
+    limit through the full re-sort and fail with E_DEPTH.  The same context
+    must then build a good input's tree and lists exactly as the oracle."""
+    rng = np.random.default_rng(21)
+    good = rng.random((11, 3))
+    bad = np.concatenate([np.zeros((10, 3)), [[1.0, 1.0, 1.0]]])
+    with vg.Gravity(11, 1, 4, "f32") as gr:  # threshold 2
+        for x, ok in ((good, True), (bad, False), (good, True)):
+            gr.upload(x)
+            if ok:
+                rt, rl = orc.tree_and_lists(x, 1)
+                assert_tree_equal(gr.tree(), rt)
+                assert_lists_equal(gr.lists(), rl)
+            else:
+                with pytest.raises(vg.GravityError) as e:
+                    gr.accelerations()
+                assert e.value.code == vg.E_DEPTH
+
+
 def test_deterministic(vg, orc):
     x, _, _ = orc.problem_initialize(30000, 8, 1e-3)
     outs = []
