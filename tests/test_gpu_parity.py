@@ -378,6 +378,28 @@ def test_depth_error_then_recovery(vg, orc):
                 assert e.value.code == vg.E_DEPTH
 
 
+def test_nonfinite_then_recovery(vg, orc):
+    """Split mode, second tree on: a non-finite coordinate is caught by the
+    device-driven build (E_INPUT) before any force work is queued on the
+    second stream; the same context then gives the forces of a fresh one,
+    bit for bit."""
+    x, _, _ = orc.problem_initialize(20000, 3, 1e-3)
+    bad = x.copy()
+    bad[123, 1] = np.nan
+    with vg.Gravity(len(x), 64, 8, "split", eps=1e-3) as gr:
+        gr.upload(x)
+        gr.accelerations()
+        gr.upload(bad)
+        with pytest.raises(vg.GravityError) as e:
+            gr.accelerations()
+        assert e.value.code == vg.E_INPUT
+        gr.upload(x)
+        a = gr.accelerations()
+    with vg.Gravity(len(x), 64, 8, "split", eps=1e-3) as gr:
+        gr.upload(x)
+        assert np.array_equal(gr.accelerations(), a)
+
+
 def test_deterministic(vg, orc):
     x, _, _ = orc.problem_initialize(30000, 8, 1e-3)
     outs = []
