@@ -400,6 +400,24 @@ def test_nonfinite_then_recovery(vg, orc):
         assert np.array_equal(gr.accelerations(), a)
 
 
+def test_two_streams_equal_one(vg, orc):
+    """Split mode runs the lists and P2P on a second stream unless per-phase
+    timing is on; both schedules must give the same forces bit for bit, over
+    several evaluations and RK4 steps of one context."""
+    x, v, _ = orc.problem_initialize(50000, 4, 1e-3)
+    runs = []
+    for timing in (False, True):
+        with vg.Gravity(len(x), 64, 12, "split", eps=1e-3) as gr:
+            gr.set_timing(timing)
+            gr.upload(x, v)
+            a0 = gr.accelerations()
+            gr.step(1e-3, 2)
+            runs.append((a0, gr.accelerations(), gr.download()))
+    (a0, a1, (x1, v1)), (b0, b1, (y1, w1)) = runs
+    assert np.array_equal(a0, b0) and np.array_equal(a1, b1)
+    assert np.array_equal(x1, y1) and np.array_equal(v1, w1)
+
+
 def test_deterministic(vg, orc):
     x, _, _ = orc.problem_initialize(30000, 8, 1e-3)
     outs = []
