@@ -11,7 +11,7 @@
 //
 //   dropin N s f64|f32|split [epsilon]
 //
-// With "precision" it does what main.cpp:33-38 does when built with
+// With "precision" it does what main.cpp:33-36 does when built with
 // ENABLE_DIFFERENT_PRECISION (experiment/different_precision.sh), again with
 // gravity::Solver replaced by gravity::GpuSolver: the reference's
 // Benchmark::benchmark_different_precision (benchmark.hpp:31-87) runs the
@@ -48,7 +48,7 @@ static void run(const std::size_t n, const std::size_t s, const std::optional<F>
     gravity::Benchmark<F, F, F, F, F, F>().benchmark(solver, n, s, eps);
 }
 
-// main.cpp:33-38 with gravity::Solver = gravity::GpuSolver; opt_ref sets the
+// main.cpp:33-36 with gravity::Solver = gravity::GpuSolver; opt_ref sets the
 // precision of the FP32 reference solver (f32 or split)
 template <class M, class T, class X, class V, class A, class G>
 static void run_precision(const std::size_t n, const std::size_t s, gravity::GpuSolverOptions opt_ref) {

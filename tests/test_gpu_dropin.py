@@ -16,7 +16,7 @@ CPU alone, 2 of 12 runs at 16 threads ended in a segfault or std::bad_alloc;
 16 of 16 single-threaded runs were clean).
 
 test_reference_different_precision: the same driver in its "precision" mode
-does what the reference's main.cpp:33-38 does when built with
+does what the reference's main.cpp:33-36 does when built with
 ENABLE_DIFFERENT_PRECISION: Benchmark::benchmark_different_precision
 (benchmark.hpp:31-87) runs GpuSolver over one of main.cpp's type sets (double;
 float with a double time type) against GpuSolver<float, ...> as its reference,
