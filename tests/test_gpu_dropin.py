@@ -55,7 +55,7 @@ def test_reference_benchmark_hook(mode):
     r = subprocess.run([DROPIN, "4096", "10", mode], capture_output=True, text=True, timeout=240, env=env)
     print(r.stdout, r.stderr)
     assert r.returncode == 0, r.stderr
-    m = re.search(r"α:(\S+) c:(\S+)", r.stdout)  # benchmark.hpp:160
+    m = re.search(r"α:(\S+) c:(\S+)", r.stdout)  # benchmark.hpp:161
     assert m, r.stdout
     assert "solver has passed the benchmark test" in r.stdout, r.stdout
 
