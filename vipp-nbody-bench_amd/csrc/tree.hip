@@ -422,6 +422,46 @@ __global__ __launch_bounds__(1024) void k_flag_write_lv(const uint8_t *__restric
     if (f && pos < cap_out) out[pos] = static_cast<int32_t>(a.base + i);
 }
 
+// The three kernels above in one block, for levels with few flags: ordered
+// compaction in chunks of 1024, then the level table updated as
+// k_count_scan_lv does.
+__global__ __launch_bounds__(1024) void k_compact1_lv(const uint8_t *__restrict__ flag, uint32_t *lv, int d, int32_t *__restrict__ out,
+                                                      int64_t cap_out, bool last) {
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t carry;
+    const LvCmp a = lv_cmp(lv, d);  // every thread reads lv before the barrier; thread 0 writes it only at the end
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    const unsigned lane = threadIdx.x & 63;
+    for (int64_t off = 0; off < a.nflag; off += 1024) {
+        const int64_t i = off + threadIdx.x;
+        const bool f = i < a.nflag && flag[i];
+        const uint64_t bal = __ballot(f);
+        if (lane == 0) wsum[threadIdx.x >> 6] = __popcll(bal);
+        __syncthreads();
+        uint32_t wb = 0;
+        for (unsigned w = 0; w < (threadIdx.x >> 6); ++w) wb += wsum[w];
+        const uint32_t c0 = carry;
+        const int64_t pos = a.out0 + c0 + wb + __popcll(bal & ((1ull << lane) - 1));
+        if (f && pos < cap_out) out[pos] = static_cast<int32_t>(a.base + i);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t tot = 0;
+            for (int w = 0; w < 16; ++w) tot += wsum[w];
+            carry = c0 + tot;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const uint32_t tot = carry;
+        const uint32_t nint = lv[LV_I + d + 1] - lv[LV_I + d];
+        lv[LV_I + d + 2] = lv[LV_I + d + 1] + tot;
+        lv[d + 2] = lv[d + 1] + 8u * nint;
+        if (lv[LV_I + d + 2] > cap_out) lv[LV_OVF] |= 1u;
+        if (last && tot > 0) lv[LV_OVF] |= 2u;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_leaf_flags_lv(const int32_t *__restrict__ child, const uint32_t *__restrict__ count, const uint32_t *lv,
                                                        int last, int64_t cap, uint8_t *__restrict__ flag) {
     const int64_t c = blockIdx.x * 256ll + threadIdx.x;
@@ -731,9 +771,13 @@ static int tree_build_lv(Ctx &c, int SH, uint64_t *&k0, uint32_t *&v0, uint64_t 
         const unsigned gc = static_cast<unsigned>(ceil_div(nflag[d], 1024));
         k_expand_lv<<<ceil_div(nflag[d], 256), 256, 0, st>>>(d, t.lv, t.int_ids, cap, k0, c.thr, t.start, t.count, t.child, t.parent, t.octant,
                                                               t.level, t.bmin, t.bmax, t.grid, t.anc, t.flags);
-        k_flag_counts_lv<<<gc, 1024, 0, st>>>(t.flags, t.lv, d, c.cmp_cnt);
-        k_count_scan_lv<<<1, 1024, 0, st>>>(c.cmp_cnt, static_cast<int>(gc), t.lv, d, cap, d == SH - 1);
-        k_flag_write_lv<<<gc, 1024, 0, st>>>(t.flags, t.lv, d, c.cmp_cnt, t.int_ids, cap);
+        if (nflag[d] <= 8192) {  // one launch instead of three
+            k_compact1_lv<<<1, 1024, 0, st>>>(t.flags, t.lv, d, t.int_ids, cap, d == SH - 1);
+        } else {
+            k_flag_counts_lv<<<gc, 1024, 0, st>>>(t.flags, t.lv, d, c.cmp_cnt);
+            k_count_scan_lv<<<1, 1024, 0, st>>>(c.cmp_cnt, static_cast<int>(gc), t.lv, d, cap, d == SH - 1);
+            k_flag_write_lv<<<gc, 1024, 0, st>>>(t.flags, t.lv, d, c.cmp_cnt, t.int_ids, cap);
+        }
     }
     VG_LAUNCH_CHECK();
     // non-empty leaves among the lv[SH + 1] cells
