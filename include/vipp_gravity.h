@@ -70,7 +70,10 @@ int vipp_grav_download(vipp_grav_ctx *ctx, double *x, double *v);
 /* nsteps classic RK4 steps of size dt (SolverRef::solve -> runge_kutta,
  * solver_ref.hpp:17-27,36-104); each step rebuilds tree and lists at every
  * stage as SolverRef::acceleration does (solver_ref.hpp:164-177).  Returns after
- * the device work has finished. */
+ * the device work has finished.  If a stage fails (E_INPUT for a non-finite
+ * stage position, E_DEPTH), x and v are put back to the state at the start of
+ * the failed step (the steps before it are kept) and tree, lists and
+ * expansions are rebuilt from that state by the next call that needs them. */
 int vipp_grav_step(vipp_grav_ctx *ctx, double dt, uint32_t nsteps);
 
 /* One force evaluation at the current positions (SolverRef::acceleration,
@@ -163,6 +166,16 @@ int vipp_grav_leaf_inputs(vipp_grav_ctx *ctx, int64_t *n, float *x, float *norm,
  * order-p_fallback far field instead (INFINITY: never); they are counted in
  * vipp_grav_counts()[6]. */
 int vipp_grav_set_closure(vipp_grav_ctx *ctx, const float *params, const float *io, double bound, uint32_t p_fallback);
+
+/* The reference's initial conditions, Problem::initialize
+ * (gravity/include/problem.hpp:188-211) for the double type set: n particles
+ * of a Plummer sphere from one mt19937_64 stream seeded with `seed`, positions
+ * and velocities into x and v (3n doubles each), bit for bit the reference's.
+ * epsilon < 0 selects the reference's default softening 0.63 n^-0.22
+ * (problem.hpp:290-292), which also truncates the mass fraction drawn; the
+ * value used goes to *epsilon_used (may be NULL).  Host code: no device is
+ * touched.  E_INVALID for n = 0, a NULL array or epsilon >= 1. */
+int vipp_grav_problem_initialize(uint64_t n, uint64_t seed, double epsilon, double *x, double *v, double *epsilon_used);
 
 /* Device pointers of the FP64 state (3n each) for callers that keep data
  * resident on the GPU; valid until destroy. */

@@ -400,6 +400,46 @@ def test_nonfinite_then_recovery(vg, orc):
         assert np.array_equal(gr.accelerations(), a)
 
 
+@pytest.mark.parametrize("mode", ["f64", "split"])
+def test_failed_step_restores_state(vg, orc, mode):
+    """A stage of an RK4 step whose Euler state overflows (x0 + v dt/2 = inf)
+    fails with E_INPUT after the first stage's tree, lists and forces were
+    built.  x and v must be put back to the state the failed step started
+    from, and every query after it must describe that state: tree, lists and
+    work counts equal to a fresh context holding it, never the stale tree of
+    the failed stage."""
+    x, v, _ = orc.problem_initialize(3000, 5, 1e-3)
+    with vg.Gravity(len(x), 8, 4, mode, eps=1e-3) as gr:
+        gr.upload(x, v)
+        gr.step(1e-3, 1)  # one good step: the context holds a tree
+        x1, v1 = gr.download()
+        vbad = v1.copy()
+        vbad[7] = [1e300, 0.0, 0.0]
+        gr.upload(x1, vbad)
+        with pytest.raises(vg.GravityError) as e:
+            gr.step(1e10, 1)  # stage 1 position 1e300 * 5e9 overflows
+        assert e.value.code == vg.E_INPUT
+        x2, v2 = gr.download()
+        assert np.array_equal(x2, x1) and np.array_equal(v2, vbad)
+        t, l = gr.tree(), gr.lists()
+        counts = gr.counts()
+    rt, rl = orc.tree_and_lists(x1, 8)
+    assert_tree_equal(t, rt)
+    assert_lists_equal(l, rl)
+    with vg.Gravity(len(x), 8, 4, mode, eps=1e-3) as gr:
+        gr.upload(x1, vbad)
+        assert gr.counts() == counts
+
+
+def test_upload_checks_sizes(vg):
+    with vg.Gravity(100, 4, 4, "f32") as gr:
+        x = np.random.default_rng(0).random((100, 3))
+        with pytest.raises(ValueError):
+            gr.upload(x, masses=np.ones(99))
+        with pytest.raises(ValueError):
+            gr.upload(x[:99])
+
+
 def test_two_streams_equal_one(vg, orc):
     """Split mode runs the lists and P2P on a second stream unless per-phase
     timing is on; both schedules must give the same forces bit for bit, over

@@ -53,3 +53,18 @@ def test_c_header_is_plain_c():
     r = subprocess.run(["gcc", "-std=c99", "-fsyntax-only", "-Wall", "-Wextra", "-Werror", "-x", "c", os.path.join(ROOT, "include", "vipp_gravity.h")],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_problem_initialize_matches_reference_stream(golden):
+    """The library's generator of the reference's initial conditions (host
+    code) against vectors the reference's own Problem::initialize wrote
+    (problem.hpp:188-211, default softening problem.hpp:290-292): bit for bit."""
+    import numpy as np
+    import vipp_gravity as vgm
+    g = golden("problem_init")
+    for n, seed, eps in ((1000, 0, None), (257, 7, 1e-3)):
+        x, v, e = vgm.problem_initialize(n, seed, eps)
+        assert np.array_equal(x, g[f"x_{n}_{seed}"]) and np.array_equal(v, g[f"v_{n}_{seed}"])
+        assert e == g[f"eps_{n}_{seed}"]
+    with pytest.raises(vgm.GravityError):
+        vgm.problem_initialize(0)

@@ -19,6 +19,9 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "lib", "libvipp_gravity.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 SOURCES = ["api.hip", "tree.hip", "sort.hip", "fmm.hip", "p2p.hip", "rk4.hip", "m2l_mfma.hip", "direct.hip", "mlp.hip", "closure.hip"]
+# host-only sources, compiled by the host compiler the reference is built with
+# (g++ without FMA contraction): problem.cpp must reproduce its libm calls
+HOST_SOURCES = ["problem.cpp"]
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wall", "-Wno-unused-function", "-munsafe-fp-atomics"]
 
@@ -32,11 +35,21 @@ def _compile(src: str) -> str:
     return obj
 
 
+def _compile_host(src: str) -> str:
+    obj = os.path.join(OBJ, src.replace(".cpp", ".o"))
+    cmd = [os.environ.get("CXX", "g++"), "-std=c++17", "-O2", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra", "-I",
+           os.path.join(os.path.dirname(HERE), "include"), "-c", os.path.join(CSRC, src), "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"host compile failed on {src}:\n{r.stderr}")
+    return obj
+
+
 def build(jobs: int = 8) -> str:
     os.makedirs(OBJ, exist_ok=True)
     os.makedirs(os.path.dirname(LIB), exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+        objs = list(ex.map(_compile, SOURCES)) + list(ex.map(_compile_host, HOST_SOURCES))
     tmp = LIB + ".tmp"
     r = subprocess.run([HIPCC, "-shared", "--offload-arch=gfx950", "-o", tmp, *objs], capture_output=True, text=True)
     if r.returncode != 0:

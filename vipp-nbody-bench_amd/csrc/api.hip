@@ -116,6 +116,10 @@ static void evaluate_overlapped(Ctx &c, bool potential) {
 
 void evaluate(Ctx &c, bool potential) {
     join_p2p(c);  // a P2P left pending by an evaluation that threw must end before the tree changes
+    // the tree, lists and expansions are rebuilt in place below: until this
+    // evaluation has finished, nothing of the previous one may be exported
+    c.evaluated = false;
+    c.t.ref_valid = false;
     bool used[PH_COUNT] = {};
     if (!c.timing && c.stream2 && c.prec == 2) {
         tree_build(c);
@@ -473,23 +477,40 @@ int vipp_grav_step(vipp_grav_ctx *ctx, double dt, uint32_t nsteps) {
         need(std::isfinite(dt), VIPP_GRAV_E_INVALID, "non-finite dt");
         VG_HIP(hipSetDevice(c.device));
         for (uint32_t s = 0; s < nsteps; ++s) {
-            bool used[vg::PH_COUNT] = {};
-            used[vg::PH_RK4] = true;
             vg::rk4_init(c);
-            for (int stage = 0; stage < 4; ++stage) {
-                vg::evaluate(c, false);
-                vg::phase_begin(c, vg::PH_RK4);
-                vg::rk4_stage(c, stage, dt);
-                vg::phase_end(c, vg::PH_RK4);
-                if (c.timing) {
-                    VG_HIP(hipStreamSynchronize(c.stream));
-                    float ms = 0;
-                    VG_HIP(hipEventElapsedTime(&ms, c.ev[2 * vg::PH_RK4], c.ev[2 * vg::PH_RK4 + 1]));
-                    c.phase_ms[vg::PH_RK4] += ms;
+            try {
+                for (int stage = 0; stage < 4; ++stage) {
+                    vg::evaluate(c, false);
+                    vg::phase_begin(c, vg::PH_RK4);
+                    vg::rk4_stage(c, stage, dt);
+                    vg::phase_end(c, vg::PH_RK4);
+                    if (c.timing) {
+                        VG_HIP(hipStreamSynchronize(c.stream));
+                        float ms = 0;
+                        VG_HIP(hipEventElapsedTime(&ms, c.ev[2 * vg::PH_RK4], c.ev[2 * vg::PH_RK4 + 1]));
+                        c.phase_ms[vg::PH_RK4] += ms;
+                    }
                 }
+            } catch (...) {
+                // a stage state the tree cannot hold (E_INPUT, E_DEPTH): put back
+                // the state this step started from, so x and v hold the last
+                // completed step and the next query rebuilds from it
+                const size_t n3 = 3 * static_cast<size_t>(c.n);
+                (void)hipMemcpyAsync(c.x.p, c.x0.p, n3 * sizeof(double), hipMemcpyDeviceToDevice, c.stream);
+                (void)hipMemcpyAsync(c.v.p, c.v0.p, n3 * sizeof(double), hipMemcpyDeviceToDevice, c.stream);
+                (void)hipStreamSynchronize(c.stream);
+                c.evaluated = false;
+                c.t.ref_valid = false;
+                throw;
             }
         }
         VG_HIP(hipStreamSynchronize(c.stream));
+        // the last evaluation saw the stage-3 positions, not the stepped ones:
+        // tree and list queries after a step rebuild at the new positions
+        if (nsteps) {
+            c.evaluated = false;
+            c.t.ref_valid = false;
+        }
     });
 }
 

@@ -76,6 +76,7 @@ def _declare(lib):
         "vipp_mlp_forward": [i32, vp, vp, i64, vp],
         "vipp_mlp_loss_grad": [i32, vp, vp, vp, vp, i64, P(C.c_float), vp],
         "vipp_mlp_train": [i32, vp, vp, vp, vp, i64, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, vp],
+        "vipp_grav_problem_initialize": [C.c_uint64, C.c_uint64, d, vp, vp, P(d)],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -83,6 +84,20 @@ def _declare(lib):
         f.restype = C.c_int
     lib.vipp_grav_last_error.argtypes = [vp]
     lib.vipp_grav_last_error.restype = C.c_char_p
+
+
+def problem_initialize(n, seed=0, eps=None):
+    """The reference's Problem::initialize stream (problem.hpp:188-211) for n
+    particles: (x, v, epsilon), x and v (n, 3) float64.  eps None selects the
+    reference's default softening 0.63 n^-0.22.  Host code in the library."""
+    lib = library()
+    x = np.zeros(3 * int(n))
+    v = np.zeros(3 * int(n))
+    e = C.c_double()
+    rc = lib.vipp_grav_problem_initialize(int(n), int(seed), -1.0 if eps is None else float(eps), _ptr(x), _ptr(v), C.byref(e))
+    if rc != 0:
+        raise GravityError(rc, "problem_initialize: n must be > 0 and epsilon < 1")
+    return x.reshape(-1, 3), v.reshape(-1, 3), e.value
 
 
 def declared_symbols() -> list[str]:
@@ -161,8 +176,11 @@ class Gravity:
     def upload(self, x, v=None, masses=None):
         x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
         v = np.zeros_like(x) if v is None else np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
-        assert x.size == 3 * self.n and v.size == 3 * self.n
-        m = None if masses is None else np.ascontiguousarray(masses, dtype=np.float64)
+        if x.size != 3 * self.n or v.size != 3 * self.n:
+            raise ValueError(f"x and v must hold 3 * {self.n} values, got {x.size} and {v.size}")
+        m = None if masses is None else np.ascontiguousarray(masses, dtype=np.float64).reshape(-1)
+        if m is not None and m.size != self.n:
+            raise ValueError(f"masses must hold {self.n} values, got {m.size}")
         self._check(self.lib.vipp_grav_upload(self.h, _ptr(x), _ptr(v), _ptr(m)))
 
     def download(self):
