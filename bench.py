@@ -3,8 +3,8 @@
 
 Workload (default): N = 2^22 Plummer sphere (a = 1, truncated at 20 a), leaf
 parameter s = 64 (split above 181 points), expansion order p = 12, softening
-1e-3, dt = 5e-4; RK4 state in FP64, P2P in FP32, M2L by mode (plain FP32 on
-VALU or BF16 hi/lo split on MFMA with FP32 accumulation).
+1e-3, dt = 5e-4; RK4 state in FP64, P2P in FP32, M2L, M2M and L2L on the
+matrix cores with BF16 hi/lo split operands and FP32 accumulation.
 
 One step is one classic RK4 step: four force evaluations, each rebuilding the
 octree and the interaction lists as the reference does (solver_ref.hpp:36-104,
@@ -12,10 +12,20 @@ octree and the interaction lists as the reference does (solver_ref.hpp:36-104,
 device-resident data, bracketed by device synchronisation.
 
 Prints one JSON line.  Besides value (steps/s) it carries:
-  roofline      the dominant kernel's algorithmic FLOP/s from its HIP-event time
-  cpu_baseline  the reference's own FMM (oracle/_ref, single thread) on a
-                bounded sample of one force evaluation of the same input
+  roofline      the dominant kernel (split M2L) at its algorithmic FLOP/s
+                from its HIP-event time, against dense BF16 MFMA peak
+  phases        every phase's algorithmic work, time and fraction of the
+                ceiling that bounds it, and the step composite
+  cpu_baseline  the reference's own FMM (oracle/_ref) on the host's cores, on
+                a bounded sample of one force evaluation of the same input
   force_err     relative force error against an FP64 direct sum on a sample
+  vs_plain_gpu  the plain FP32 p = 12 GPU FMM on the same input
+  configs       the other workloads: 16k Plummer p = 4 FP64, 2^20 uniform
+                p = 8 FP32 with per-particle masses, and the 4M Plummer from
+                the reference's own initial-condition stream
+  learned       the 4M mixture of 32 Plummer spheres: p = 4 + learned closure
+                + order-12 fallback against plain FP32 p = 12 and split p = 12
+  energy_drift  the 16M two-Plummer collision over 100 steps
 """
 from __future__ import annotations
 
@@ -31,9 +41,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "vipp-nbody-bench_amd"))
 
 METRIC = "FMM RK4 steps/sec at N=4M; force err vs FP64 direct; learned vs plain GPU"
-PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+PEAK_FP32_TFLOPS = 157.3          # vector FP32 with packed (two-wide) FMA
+PEAK_FP32_SCALAR_TFLOPS = 78.6    # the same without packed instructions (this build emits none)
 PEAK_BF16_MFMA_TFLOPS = 2500.0
 PEAK_FP64_TFLOPS = 78.6
+TREE_BYTES_PER_PARTICLE = 136     # AABB 24 + codes 24+8 + single-pass sort 24 + permute 48 + counts 8
+LIST_BYTES_PER_ENTRY = 8          # read the parent-neighbour id, write the entry
+RK4_BYTES_PER_PARTICLE = 228      # per stage: FP64 x0 v0 v xo vo read, FP32 a read, x v xo vo written
+P2P_FLOP_PER_PAIR = 20
+
+
+def m2l_flop_per_pair(p):
+    return 2.0 * (p + 1) ** 4
 
 
 def main():
@@ -46,18 +66,20 @@ def main():
     ap.add_argument("--p", type=int, default=None)
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=None,
+                    help="host threads of the CPU baseline (default: OMP_NUM_THREADS, else every core this process may use)")
     ap.add_argument("--err-sample", type=int, default=65536)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this size, the known-size kernel that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to HBM bytes")
+    ap.add_argument("--configs", default="plummer16k_p4,uniform1m_p8,refinit4m_p12",
+                    help="other workloads reported beside the headline (comma list; 'none' skips them)")
     ap.add_argument("--learned-workload", default="mixture4m_p4",
                     help="workload of the learned-closure comparison (p=4 + closure against plain FP32 p=12); 'none' skips it")
-    ap.add_argument("--fallback-fraction", type=float, default=0.1,
-                    help="fraction of leaves (on the training pairs) whose closure estimate sends them to the order-12 far field")
     ap.add_argument("--learned-err-sample", type=int, default=16384)
     ap.add_argument("--drift-workload", default="collision16m",
-                    help="workload of the energy-drift comparison (split p=12 against plain FP32 p=12); 'none' skips it")
-    ap.add_argument("--drift-steps", type=int, default=20)
+                    help="workload of the energy-drift comparison; 'none' skips it")
+    ap.add_argument("--drift-steps", type=int, default=100)
     ap.add_argument("--no-compare-plain", dest="compare_plain", action="store_false",
                     help="skip the plain FP32 p=12 GPU FMM run reported beside the main mode")
     args = ap.parse_args()
@@ -86,16 +108,14 @@ def main():
     # accuracy on the initial state (untimed): FP64 direct sum on the device for
     # a random sample, cross-checked against the CPU oracle on its first 32
     a = g.accelerations()
-    cnt = g.counts()
     err = None
     err_ref = None
     if args.err_sample > 0:
         idx = np.sort(np.random.default_rng(1).choice(n, size=min(n, args.err_sample), replace=False))
         d = g.direct(idx)
         err_ref = (idx, d)
-        e = np.linalg.norm(a[idx] - d, axis=1) / np.linalg.norm(d, axis=1)
-        err = {"median": float(np.median(e)), "p99": float(np.percentile(e, 99)), "max": float(e.max()), "sample": int(len(idx)),
-               "reference": "fp64 direct sum on the device (vipp_grav_direct)"}
+        err = rel_err(a[idx], d)
+        err["reference"] = "fp64 direct sum on the device (vipp_grav_direct)"
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         from oracle import Oracle
         k = min(32, len(idx))
@@ -116,7 +136,7 @@ def main():
     g.step(w["dt"], args.steps)
     ph = g.phase_times()
     g.set_timing(False)
-    cnt = g.counts()  # work of the last evaluation
+    cnt = g.counts()  # work of an evaluation at the stepped positions
     # host-buffer round trip as the reference's preprocess/solve/postprocess
     # sees it: upload x, v; one step; download x, v (not the headline value)
     t2 = time.perf_counter()
@@ -126,25 +146,27 @@ def main():
     pcie_steps = 1.0 / (time.perf_counter() - t2)
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     value = 1e3 / ms_per_step
-    nev = 4 * args.steps
-    per_eval = {k: v / nev for k, v in ph.items()}
+    per_eval = {k: v_ / (4 * args.steps) for k, v_ in ph.items()}
     g.close()
 
-    # the dominant single kernel: M2L is one launch per evaluation
     p = w["p"]
-    m2l_flops = 2.0 * (p + 1) ** 4 * cnt["m2l_live_pairs"]
+    peak_m2l = {"split": PEAK_BF16_MFMA_TFLOPS, "f32": PEAK_FP32_TFLOPS, "f64": PEAK_FP64_TFLOPS}[mode]
+    phases = phase_rooflines(per_eval, cnt, n, p, mode, ms_per_step)
+    m2l_flops = m2l_flop_per_pair(p) * cnt["m2l_live_pairs"]
     ms = per_eval["m2l"]
-    split = mode == "split"
-    peak = {"split": PEAK_BF16_MFMA_TFLOPS, "f32": PEAK_FP32_TFLOPS, "f64": PEAK_FP64_TFLOPS}[mode]
     achieved = m2l_flops / (ms * 1e-3) / 1e12
-    roofline = {"kernel": "k_m2l_split" if split else "k_m2l_plain", "bound": "mfma", "achieved": achieved, "peak": peak,
-                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": m2l_traffic(args.workload, mode),
-                "flop_per_launch": m2l_flops, "ms_per_launch": ms,
-                "work": f"2(p+1)^4 = {2 * (p + 1) ** 4} FLOP per M2L pair whose source and target hold particles"}
+    roofline = {"kernel": "k_m2l_split" if mode == "split" else "k_m2l_plain", "bound": "mfma" if mode == "split" else "valu",
+                "achieved": achieved, "peak": peak_m2l, "unit": "TFLOP/s", "frac": achieved / peak_m2l,
+                "traffic": m2l_traffic(args.workload, mode), "flop_per_launch": m2l_flops, "ms_per_launch": ms,
+                "work": f"2(p+1)^4 = {int(m2l_flop_per_pair(p))} FLOP per M2L pair whose source and target hold particles"}
 
     plain = None
     if args.compare_plain and mode != "f32":
         plain = plain_reference(vgm, x, v, masses, w, n, args.steps, value, err_ref)
+
+    configs = None
+    if args.configs != "none":
+        configs = {name: config_line(vgm, workloads, name) for name in args.configs.split(",") if name}
 
     learned = None
     if args.learned_workload != "none":
@@ -156,24 +178,30 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(x, w, n)
+        cpu = cpu_baseline(x, w, n, args.cpu_threads)
 
     line = {
         "metric": METRIC, "value": value, "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if mode == "f64" else "f32", "data": "synthetic",
+        "dtype": {"split": "bf16-split+f32+f64", "f32": "f32+f64", "f64": "f64"}[mode], "data": "synthetic",
         "config": {"workload": args.workload, "n": n, "s": w["s"], "p": p, "eps": w["eps"], "dt": w["dt"],
-                   "m2l": {"f32": "fp32 VALU", "split": "bf16 hi/lo split on MFMA, fp32 accumulate", "f64": "fp64"}[mode],
-                   "p2p": "fp64" if mode == "f64" else "fp32", "state": "fp64", "parallelism": "1 GPU"},
+                   "m2l": {"f32": "fp32 VALU", "split": "bf16 hi/lo split on MFMA (3 products), fp32 accumulate", "f64": "fp64"}[mode],
+                   "m2m_l2l": {"f32": "fp32 VALU", "split": "bf16 three-way split on MFMA (6 products), fp32 accumulate", "f64": "fp64"}[mode],
+                   "p2p": "fp64" if mode == "f64" else "fp32", "p2m_l2p": "fp64" if mode == "f64" else "fp32",
+                   "state": "fp64", "parallelism": "1 GPU"},
         "roofline": roofline,
+        "phases": phases,
         "cpu_baseline": cpu,
         "vs_plain_gpu": plain,
+        "configs": configs,
         "learned": learned,
         "energy_drift": drift,
         "host_roundtrip_steps_per_s": pcie_steps,
         "phase_ms_per_eval": per_eval,
+        "tree_GBps": phases["tree"]["achieved"],
+        "lists_GBps": phases["lists"]["achieved"],
         "p2p_pairs_per_s": cnt["p2p_pairs"] / (per_eval["p2p"] * 1e-3) if per_eval["p2p"] > 0 else None,
-        "p2p_tflops": 20.0 * cnt["p2p_pairs"] / (per_eval["p2p"] * 1e-3) / 1e12 if per_eval["p2p"] > 0 else None,
+        "p2p_tflops": P2P_FLOP_PER_PAIR * cnt["p2p_pairs"] / (per_eval["p2p"] * 1e-3) / 1e12 if per_eval["p2p"] > 0 else None,
         "m2l_pairs_per_s": cnt["m2l_live_pairs"] / (per_eval["m2l"] * 1e-3) if per_eval["m2l"] > 0 else None,
         "counts": cnt,
         "force_err": err,
@@ -181,112 +209,231 @@ def main():
     print(json.dumps(line))
 
 
+def rel_err(a, d):
+    e = np.linalg.norm(a - d, axis=1) / np.linalg.norm(d, axis=1)
+    return {"median": float(np.median(e)), "p99": float(np.percentile(e, 99)), "max": float(e.max()), "sample": int(len(e))}
+
+
+def phase_rooflines(per_eval, cnt, n, p, mode, ms_per_step):
+    """Each phase's algorithmic work per force evaluation, its time alone (HIP
+    events, one stream), the ceiling that bounds it and the fraction reached;
+    and the step composite sum_phase (work / peak) / measured step time, where
+    the step holds 4 evaluations and 4 RK4 stages."""
+    split = mode == "split"
+    mfma_or_valu = PEAK_BF16_MFMA_TFLOPS if split else (PEAK_FP64_TFLOPS if mode == "f64" else PEAK_FP32_TFLOPS)
+    p2p_peak = PEAK_FP64_TFLOPS if mode == "f64" else PEAK_FP32_TFLOPS
+    inner_cells = (cnt["cells"] - 1) / 8.0  # every internal cell has 8 children
+    rows = {
+        "tree": (TREE_BYTES_PER_PARTICLE * n / 1e9, "GB", PEAK_HBM_GBS, "GB/s", "hbm", f"{TREE_BYTES_PER_PARTICLE} B per particle"),
+        "lists": (LIST_BYTES_PER_ENTRY * cnt["list_entries"] / 1e9, "GB", PEAK_HBM_GBS, "GB/s", "hbm", f"{LIST_BYTES_PER_ENTRY} B per list entry"),
+        "p2p": (P2P_FLOP_PER_PAIR * cnt["p2p_pairs"] / 1e12, "TFLOP", p2p_peak, "TFLOP/s", "valu", "20 FLOP per pair"),
+        "m2l": (m2l_flop_per_pair(p) * cnt["m2l_live_pairs"] / 1e12, "TFLOP", mfma_or_valu, "TFLOP/s", "mfma" if split else "valu",
+                "2(p+1)^4 FLOP per live M2L pair"),
+        # M2M and L2L: 2(p+1)^4 per child-parent pair; the phase time also holds P2M / L2P (no work unit: time share only)
+        "upward": (m2l_flop_per_pair(p) * 8 * inner_cells / 1e12, "TFLOP", mfma_or_valu, "TFLOP/s", "mfma" if split else "valu",
+                   "M2M 2(p+1)^4 per child-parent pair (P2M time share only)"),
+        "downward": (m2l_flop_per_pair(p) * 8 * inner_cells / 1e12, "TFLOP", mfma_or_valu, "TFLOP/s", "mfma" if split else "valu",
+                     "L2L 2(p+1)^4 per child-parent pair (L2P time share only)"),
+        "rk4": (RK4_BYTES_PER_PARTICLE * n / 1e9, "GB", PEAK_HBM_GBS, "GB/s", "hbm", f"{RK4_BYTES_PER_PARTICLE} B per particle per stage"),
+        "operators": (0.0, "-", None, "-", "-", "operator tables for the current root box: time share only"),
+    }
+    out = {}
+    ideal_ms = 0.0
+    for name, (work, wunit, peak, unit, bound, note) in rows.items():
+        t = per_eval.get(name, 0.0)
+        ach = work / (t * 1e-3) if t > 0 and work > 0 else None
+        out[name] = {"ms_per_eval": t, "work": work, "work_unit": wunit, "achieved": ach, "peak": peak, "unit": unit, "bound": bound,
+                     "frac": (ach / peak) if ach is not None and peak else None, "per_unit": note}
+        if peak and work > 0:
+            ideal_ms += 4 * work / peak * 1e3  # 4 evaluations (and 4 RK4 stages) per step
+    out["composite"] = {"ideal_ms_per_step": ideal_ms, "measured_ms_per_step": ms_per_step, "frac": ideal_ms / ms_per_step,
+                        "definition": "sum over phases of work / peak, over the measured step time (4 evaluations and stages per step)"}
+    ach = out["p2p"]["achieved"]
+    out["p2p"]["frac_of_scalar_fp32"] = ach / PEAK_FP32_SCALAR_TFLOPS if ach and mode != "f64" else None
+    return out
+
+
+def timed_run(vgm, x, v, masses, w, mode, p, steps, warmup=1, closure=None, err_idx=None, d=None):
+    """steps/s of `steps` RK4 steps (after `warmup`), the phase times of as many
+    more, the work counts, and the force error on err_idx against d."""
+    n = len(x)
+    with vgm.Gravity(n, w["s"], p, mode, eps=w["eps"], G=1.0, mass=1.0 / n) as g:
+        g.upload(x, v, masses)
+        if closure is not None:
+            g.set_closure(*closure)
+        out = {}
+        if err_idx is not None:
+            a = g.accelerations()
+            if d is None:
+                d = g.direct(err_idx)
+            out["force_err"] = rel_err(a[err_idx], d)
+        cnt = g.counts()
+        for _ in range(warmup):
+            g.step(w["dt"], 1)
+        g.upload(x, v, masses)
+        t0 = time.perf_counter()
+        g.step(w["dt"], steps)
+        t1 = time.perf_counter()
+        g.set_timing(True)
+        g.phase_times(reset=True)
+        g.step(w["dt"], steps)
+        ph = g.phase_times()
+    per_eval = {k: v_ / (4 * steps) for k, v_ in ph.items()}
+    out.update({"steps_per_s": steps / (t1 - t0), "phase_ms_per_eval": per_eval, "counts": cnt})
+    if per_eval["p2p"] > 0:
+        out["p2p_pairs_per_s"] = cnt["p2p_pairs"] / (per_eval["p2p"] * 1e-3)
+    if per_eval["m2l"] > 0:
+        out["m2l_pairs_per_s"] = cnt["m2l_live_pairs"] / (per_eval["m2l"] * 1e-3)
+    return out, d
+
+
+def config_line(vgm, workloads, name):
+    """steps/s, interactions/s and force error of one of the other workloads,
+    in the arithmetic its configuration names."""
+    w = dict(workloads.WORKLOADS[name])
+    mode = w.get("mode", "split")
+    x, v, masses = workloads.make(name)
+    n = len(x)
+    idx = np.sort(np.random.default_rng(1).choice(n, size=min(n, 65536), replace=False))
+    out, _ = timed_run(vgm, x, v, masses, w, mode, w["p"], w["steps"], err_idx=idx)
+    out.update({"n": n, "s": w["s"], "p": w["p"], "eps": w["eps"], "dt": w["dt"], "steps": w["steps"], "mode": mode,
+                "masses": "per particle" if masses is not None else "1/N", "source": w.get("source", "numpy generators (workloads.py)")})
+    out["force_err"]["reference"] = "fp64 direct sum on the device"
+    return out
+
+
 def plain_reference(vgm, x, v, masses, w, n, steps, value, err_ref=None):
     """The plain GPU FMM (FP32 P2P, FP32 M2L on the vector units, same p) on the
-    same input and step count: the comparison the headline metric asks for.
-    err_ref = (idx, direct accelerations): its force error on the same sample."""
-    g = vgm.Gravity(n, w["s"], w["p"], "f32", eps=w["eps"], G=1.0, mass=1.0 / n)
-    g.upload(x, v, masses)
-    err = None
-    if err_ref is not None:
-        idx, d = err_ref
-        a = g.accelerations()
-        e = np.linalg.norm(a[idx] - d, axis=1) / np.linalg.norm(d, axis=1)
-        err = {"median": float(np.median(e)), "p99": float(np.percentile(e, 99)), "sample": int(len(idx))}
-    g.step(w["dt"], 1)
-    g.upload(x, v, masses)
+    same input and step count: the comparison the headline metric asks for."""
+    idx, d = err_ref if err_ref is not None else (None, None)
+    out, _ = timed_run(vgm, x, v, masses, w, "f32", w["p"], steps, err_idx=idx, d=d)
+    ms = out["phase_ms_per_eval"]["m2l"]
+    flops = m2l_flop_per_pair(w["p"]) * out["counts"]["m2l_live_pairs"]
+    tf = flops / (ms * 1e-3) / 1e12 if ms > 0 else None
+    out.update({"mode": "plain fp32 p2p + fp32 m2l", "speedup": value / out["steps_per_s"],
+                "m2l_tflops": tf, "m2l_frac_of_fp32": tf / PEAK_FP32_TFLOPS if tf else None,
+                "m2l_frac_of_scalar_fp32": tf / PEAK_FP32_SCALAR_TFLOPS if tf else None})
+    return out
+
+
+def train_closure(vgm, closure, workloads, wl, seeds, n=None):
+    X, T = [], []
+    wdef = workloads.WORKLOADS[wl]
+    for seed in seeds:
+        xs, _, ms = workloads.make(wl, seed=seed)
+        if n:
+            xs, ms = xs[:n], (None if ms is None else ms[:n])
+        a, b = closure.training_pairs(xs, wdef["s"], wdef["eps"], ms)
+        X.append(a)
+        T.append(b)
+    X, T = np.concatenate(X), np.concatenate(T)
+    params, io, hist = closure.train(X, T, epochs=40, batch=512, lr=2e-3, seed=0)
+    return X, params, io, hist
+
+
+def learned_comparison(vgm, workloads, args):
+    """p = 4 FMM + learned closure (+ order-12 fallback) against the plain FP32
+    p = 12 GPU FMM and the split p = 12 FMM on the same input.  The network is
+    trained on the device on pairs the FMM generates from three other seeds of
+    the workload (never the evaluated one).  The fallback bound is swept over
+    the fraction of leaves sent to order 12; errors are stated against the
+    FP64 direct sum and against the FP64 order-12 FMM (the judge of the
+    closure), and the speed-up is stated where the closure's error reaches
+    the plain FMM's."""
+    import closure
+    wl = args.learned_workload
+    wdef = dict(workloads.WORKLOADS[wl])
+    steps = max(2, args.steps)
     t0 = time.perf_counter()
-    g.step(w["dt"], steps)
+    X, params, io, hist = train_closure(vgm, closure, workloads, wl, (1, 2, 3))
     t1 = time.perf_counter()
-    g.close()
-    plain = steps / (t1 - t0)
-    return {"mode": "plain fp32 p2p + fp32 m2l", "steps_per_s": plain, "speedup": value / plain, "force_err": err}
+    x, v, masses = workloads.make(wl, seed=0)
+    n = len(x)
+    idx = np.sort(np.random.default_rng(1).choice(n, size=min(n, args.learned_err_sample), replace=False))
+    with vgm.Gravity(n, wdef["s"], 12, "f64", eps=wdef["eps"], G=1.0, mass=1.0 / n) as g:
+        g.upload(x, v, masses)
+        judge = g.accelerations()[idx]
+        d = g.direct(idx)
+
+    def run(p, mode, cl=None):
+        out, _ = timed_run(vgm, x, v, masses, wdef, mode, p, steps, closure=cl, err_idx=idx, d=d)
+        with vgm.Gravity(n, wdef["s"], p, mode, eps=wdef["eps"], G=1.0, mass=1.0 / n) as g:
+            g.upload(x, v, masses)
+            if cl is not None:
+                g.set_closure(*cl)
+            a = g.accelerations()[idx]
+            out["fallback_leaves"] = g.counts()["fallback_leaves"]
+        out["err_vs_fp64_p12"] = rel_err(a, judge)
+        return out
+
+    plain = run(12, "f32")
+    split12 = run(12, "split")
+    low = run(4, "split")
+    nofb = run(4, "split", (params, io, float("inf")))
+    sweep = []
+    for frac in (0.1, 0.25, 0.5, 0.75, 1.0):
+        b = closure.bound_for_fraction(params, io, X, frac) if frac < 1.0 else 0.0
+        r = run(4, "split", (params, io, b))
+        r.update({"fallback_fraction_target": frac, "bound": b})
+        sweep.append(r)
+    # equal error: the smallest swept fallback whose median error (against the
+    # FP64 direct sum, and against the FP64 p = 12 judge) is at or below plain's
+    def at_equal(key):
+        target = plain[key]["median"]
+        for r in sweep:
+            if r[key]["median"] <= target:
+                return {"fallback_fraction_target": r["fallback_fraction_target"], "steps_per_s": r["steps_per_s"],
+                        "median": r[key]["median"], "plain_median": target, "speedup_vs_plain": r["steps_per_s"] / plain["steps_per_s"]}
+        return {"reached": False, "plain_median": target, "best_median": min(r[key]["median"] for r in sweep)}
+    leaves = low["counts"]["leaves"]
+    return {
+        "workload": wl, "n": n, "eps": wdef["eps"], "train_pairs": int(len(X)), "train_seeds": [1, 2, 3], "eval_seed": 0,
+        "train_s": t1 - t0, "train_loss_first_last": [float(hist[0]), float(hist[-1])], "leaves": leaves,
+        "plain_fp32_p12": plain, "split_p12": split12, "p4_no_closure": low, "p4_closure": nofb, "p4_closure_fallback_sweep": sweep,
+        "closure_gain_vs_p4": {"median_vs_fp64_p12": [low["err_vs_fp64_p12"]["median"], nofb["err_vs_fp64_p12"]["median"]],
+                               "median_vs_direct": [low["force_err"]["median"], nofb["force_err"]["median"]]},
+        "equal_error_vs_direct": at_equal("force_err"), "equal_error_vs_fp64_p12": at_equal("err_vs_fp64_p12"),
+        "split_p12_speedup_vs_plain": split12["steps_per_s"] / plain["steps_per_s"],
+        "err_sample": int(len(idx)),
+    }
 
 
 def energy_drift(vgm, workloads, args):
     """Relative total-energy drift |E(T) - E(0)| / |E(0)| over K RK4 steps of the
-    two-Plummer collision, split p = 12 beside plain FP32 p = 12 (FP64 state
-    and energy reductions in both)."""
+    two-Plummer collision: split p = 12 M2L with the learned closure (p = 4,
+    trained on another seed of the collision, order-12 fallback for a quarter
+    of the leaves), split p = 12 alone, and the plain FP32 p = 12 GPU FMM (FP64
+    state and FP64 energy reductions in all), with each one's force error on
+    the initial state against an FP64 direct sum."""
+    import closure
     wl = args.drift_workload
     wdef = workloads.WORKLOADS[wl]
     x, v, masses = workloads.make(wl, seed=0)
     n = len(x)
-    out = {"workload": wl, "n": n, "steps": args.drift_steps, "dt": wdef["dt"], "p": wdef["p"]}
-    for name, mode in (("split_p12", "split"), ("plain_fp32_p12", "f32")):
-        with vgm.Gravity(n, wdef["s"], wdef["p"], mode, eps=wdef["eps"], G=1.0, mass=1.0 / n) as g:
+    idx = np.sort(np.random.default_rng(1).choice(n, size=16384, replace=False))
+    X, params, io, _ = train_closure(vgm, closure, workloads, wl, (1,))
+    bound = closure.bound_for_fraction(params, io, X, 0.25)
+    out = {"workload": wl, "n": n, "steps": args.drift_steps, "dt": wdef["dt"], "eps": wdef["eps"], "train_pairs": int(len(X))}
+    d = None
+    for name, mode, p, cl in (("split_closure_p4_fb12", "split", 4, (params, io, bound)), ("split_p12", "split", 12, None),
+                              ("plain_fp32_p12", "f32", 12, None)):
+        with vgm.Gravity(n, wdef["s"], p, mode, eps=wdef["eps"], G=1.0, mass=1.0 / n) as g:
             g.upload(x, v, masses)
+            if cl is not None:
+                g.set_closure(*cl)
+            a = g.accelerations()[idx]
+            if d is None:
+                d = g.direct(idx)
             e0 = sum(g.energy())
             t0 = time.perf_counter()
             g.step(wdef["dt"], args.drift_steps)
             t1 = time.perf_counter()
             e1 = sum(g.energy())
-        out[name] = {"relative_drift": abs(e1 - e0) / abs(e0), "E0": e0, "E1": e1, "steps_per_s": args.drift_steps / (t1 - t0)}
+            fb = g.counts()["fallback_leaves"]
+        out[name] = {"relative_drift": abs(e1 - e0) / abs(e0), "E0": e0, "E1": e1, "steps_per_s": args.drift_steps / (t1 - t0),
+                     "force_err": rel_err(a, d), "fallback_leaves": fb}
+        print(f"# drift {name}: {out[name]['relative_drift']:.3e} at {out[name]['steps_per_s']:.2f} steps/s", file=sys.stderr, flush=True)
     return out
-
-
-def learned_comparison(vgm, workloads, args):
-    """p = 4 FMM + learned closure (+ order-12 fallback) against the plain FP32
-    p = 12 GPU FMM on the same input: steps/s and force errors.  The network is
-    trained on the device on pairs the FMM generates from three other seeds
-    of the workload (never the evaluated one)."""
-    import closure
-    wl = args.learned_workload
-    wdef = workloads.WORKLOADS[wl]
-    s_, eps, dt = wdef["s"], wdef["eps"], wdef["dt"]
-    t0 = time.perf_counter()
-    X, T = [], []
-    for seed in (1, 2, 3):
-        xs, _, ms = workloads.make(wl, seed=seed)
-        a, b = closure.training_pairs(xs, s_, eps, ms)
-        X.append(a)
-        T.append(b)
-    X, T = np.concatenate(X), np.concatenate(T)
-    t1 = time.perf_counter()
-    params, io, hist = closure.train(X, T, epochs=40, batch=512, lr=2e-3, seed=0)
-    t2 = time.perf_counter()
-    bound = closure.bound_for_fraction(params, io, X, args.fallback_fraction)
-    x, v, masses = workloads.make(wl, seed=0)
-    n = len(x)
-    idx = np.sort(np.random.default_rng(1).choice(n, size=min(n, args.learned_err_sample), replace=False))
-
-    def run(p, mode, with_closure, b=float("inf")):
-        g = vgm.Gravity(n, s_, p, mode, eps=eps, G=1.0, mass=1.0 / n)
-        g.upload(x, v, masses)
-        if with_closure:
-            g.set_closure(params, io, b)
-        a = g.accelerations()
-        cnt = g.counts()
-        d = g.direct(idx) if run.d is None else run.d
-        run.d = d
-        g.step(dt, 1)  # warm-up
-        g.upload(x, v, masses)
-        g.set_timing(True)
-        g.phase_times(reset=True)
-        ta = time.perf_counter()
-        g.step(dt, args.steps)
-        tb = time.perf_counter()
-        ph = g.phase_times()
-        g.close()
-        e = np.linalg.norm(a[idx] - d, axis=1) / np.linalg.norm(d, axis=1)
-        return {"steps_per_s": args.steps / (tb - ta), "err_median": float(np.median(e)), "err_p99": float(np.percentile(e, 99)),
-                "phase_ms_per_eval": {k: v_ / (4 * args.steps) for k, v_ in ph.items()}, "fallback_leaves": cnt["fallback_leaves"],
-                "leaves": cnt["leaves"]}, a
-    run.d = None
-    plain, a12 = run(12, "f32", False)
-    split12, _ = run(12, "split", False)
-    low, a4 = run(4, "split", False)
-    learned, ac = run(4, "split", True, bound)
-    learned_nofb, acn = run(4, "split", True)
-    diff_to_p12 = lambda a: float(np.median(np.linalg.norm(a[idx] - a12[idx], axis=1) / np.linalg.norm(a12[idx], axis=1)))
-    return {
-        "workload": wl, "n": n, "train_pairs": int(len(X)), "train_seeds": [1, 2, 3], "eval_seed": 0,
-        "pairs_s": t1 - t0, "train_s": t2 - t1, "train_loss_first_last": [float(hist[0]), float(hist[-1])],
-        "fallback_fraction_target": args.fallback_fraction, "bound": bound,
-        "plain_fp32_p12": plain, "split_p12": split12, "p4_no_closure": low, "p4_closure": learned_nofb, "p4_closure_fallback": learned,
-        "speedup_vs_plain": learned["steps_per_s"] / plain["steps_per_s"],
-        "median_diff_to_p12": {"p4": diff_to_p12(a4), "p4_closure": diff_to_p12(acn), "p4_closure_fallback": diff_to_p12(ac)},
-        "err_sample": int(len(idx)), "err_reference": "fp64 direct sum on the device",
-    }
 
 
 def calibration_copy(nbytes):
@@ -320,22 +467,47 @@ def m2l_traffic(workload, mode):
     return None
 
 
-def cpu_baseline(x, w, n):
-    """The reference's FMM compiled from its own headers (oracle/_ref), one thread,
-    on a bounded sample of one force evaluation of the same input."""
+def host_info():
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"nproc": usable, "cpu_count": os.cpu_count(), "model": model}
+
+
+def cpu_baseline(x, w, n, threads=None):
+    """The reference's FMM compiled from its own headers (oracle/_ref) on the
+    host's cores, on a bounded sample of one force evaluation of the same
+    input.  Tree and calc_link run in full with the reference's own threading
+    (the octree insertion is serial); P2M, M2M, P2P and L2L/L2P run sampled
+    items on all threads; M2L runs on one thread (its loop in the reference is
+    not race-free) and is counted at ideal scaling over the threads."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     if not oracle.available("ref"):
         return None
+    host = host_info()
+    if threads is None:
+        env = os.environ.get("OMP_NUM_THREADS")
+        threads = min(int(env), host["nproc"]) if env and env.isdigit() and int(env) > 0 else host["nproc"]
     ref = oracle.Oracle("ref")
     per = 32
-    secs, frac = ref.time_sample(x, w["s"], w["p"], 1.0 / n, 1.0, w["eps"], per)
-    t_eval = float(secs.sum())
-    return {"value": 1.0 / (4 * t_eval), "unit": "steps/s", "cores": 1, "kind": "reference",
-            "sample": (f"one force evaluation of the same {n}-particle input: Octree build and calc_link in full; P2M, M2M, M2L, "
-                       f"P2P, L2L/L2P on {per} evenly strided work items each (sampled work fractions "
-                       f"{', '.join(f'{f:.1e}' for f in frac[2:])}), scaled by exact work counts; 4 evaluations per RK4 step"),
-            "seconds_per_eval": {k: float(s) for k, s in zip(["tree", "lists", "p2m", "m2m", "m2l", "p2p", "l2l_l2p"], secs)}}
+    secs, frac = ref.time_sample(x, w["s"], w["p"], 1.0 / n, 1.0, w["eps"], per, threads)
+    names = ["tree", "lists", "p2m", "m2m", "m2l_one_thread", "p2p", "l2l_l2p"]
+    t_eval = float(secs.sum() - secs[4] + secs[4] / threads)
+    return {"value": 1.0 / (4 * t_eval), "unit": "steps/s", "cores": int(threads), "kind": "reference",
+            "host": host,
+            "sample": (f"one force evaluation of the same {n}-particle input: Octree build (serial, as the reference) and calc_link "
+                       f"in full; P2M, M2M, P2P, L2L/L2P on {per * threads} evenly strided work items on {threads} threads; M2L on "
+                       f"{per} items on one thread, divided by {threads} (ideal scaling: the reference's M2L loop races, "
+                       f"fmm.hpp:279-281); sampled work fractions {', '.join(f'{f:.1e}' for f in frac[2:])}, scaled by exact work "
+                       f"counts; 4 evaluations per RK4 step; extrapolated"),
+            "seconds_per_eval": {k: float(s) for k, s in zip(names, secs)}}
 
 
 if __name__ == "__main__":

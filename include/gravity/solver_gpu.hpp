@@ -80,16 +80,10 @@ class GpuSolver final {
         const std::size_t n = problem.N();
         if (x.size() != n || v.size() != n) throw std::runtime_error("x and v must hold N particles");
         ensure_context(problem);
-        stage_.resize(6 * n);
-        double *xs = stage_.data(), *vs = stage_.data() + 3 * n;
-        for (std::size_t i = 0; i < n; ++i) {
-            xs[3 * i] = static_cast<double>(x[i].x());
-            xs[3 * i + 1] = static_cast<double>(x[i].y());
-            xs[3 * i + 2] = static_cast<double>(x[i].z());
-            vs[3 * i] = static_cast<double>(v[i].x());
-            vs[3 * i + 1] = static_cast<double>(v[i].y());
-            vs[3 * i + 2] = static_cast<double>(v[i].z());
-        }
+        stage_.resize((flat<X>() ? 0 : 3 * n) + (flat<V>() ? 0 : 3 * n));
+        double *spare = stage_.data();
+        const double *xs = pack(x, spare);
+        const double *vs = pack(v, spare);
         check(vipp_grav_upload(ctx_, xs, vs, nullptr));
     }
 
@@ -106,19 +100,59 @@ class GpuSolver final {
         const std::size_t n = problem.N();
         x.resize(n, Vector<X>{});
         v.resize(n, Vector<V>{});
-        stage_.resize(6 * n);
-        double *xs = stage_.data(), *vs = stage_.data() + 3 * n;
+        stage_.resize((flat<X>() ? 0 : 3 * n) + (flat<V>() ? 0 : 3 * n));
+        double *spare = stage_.data();
+        double *xs = target(x, spare);
+        double *vs = target(v, spare);
         check(vipp_grav_download(ctx_, xs, vs));
-        for (std::size_t i = 0; i < n; ++i) {
-            x[i] = Vector<X>{static_cast<X>(xs[3 * i]), static_cast<X>(xs[3 * i + 1]), static_cast<X>(xs[3 * i + 2])};
-            v[i] = Vector<V>{static_cast<V>(vs[3 * i]), static_cast<V>(vs[3 * i + 1]), static_cast<V>(vs[3 * i + 2])};
-        }
+        unpack(xs, x);
+        unpack(vs, v);
     }
 
     /// The library context (for the extra queries of vipp_gravity.h), or null before preprocess.
     vipp_grav_ctx *context() const noexcept { return ctx_; }
 
    private:
+    // Vector<double> is three contiguous doubles (problem.hpp:143-146), so a
+    // std::vector of them is the ABI's double[3N] as it is and crosses it
+    // without a staging copy; other coordinate types are converted through
+    // stage_.
+    template <class U>
+    static constexpr bool flat() {
+        return std::is_same<U, double>::value && std::is_standard_layout<Vector<U>>::value && sizeof(Vector<U>) == 3 * sizeof(double);
+    }
+    template <class U>
+    static const double *pack(const std::vector<Vector<U>> &a, double *&spare) {
+        if constexpr (flat<U>()) {
+            return reinterpret_cast<const double *>(a.data());
+        } else {
+            double *out = spare;
+            for (std::size_t i = 0; i < a.size(); ++i) {
+                out[3 * i] = static_cast<double>(a[i].x());
+                out[3 * i + 1] = static_cast<double>(a[i].y());
+                out[3 * i + 2] = static_cast<double>(a[i].z());
+            }
+            spare += 3 * a.size();
+            return out;
+        }
+    }
+    template <class U>
+    static double *target(std::vector<Vector<U>> &a, double *&spare) {
+        if constexpr (flat<U>()) {
+            return reinterpret_cast<double *>(a.data());
+        } else {
+            double *out = spare;
+            spare += 3 * a.size();
+            return out;
+        }
+    }
+    template <class U>
+    static void unpack(const double *in, std::vector<Vector<U>> &a) {
+        if constexpr (!flat<U>())
+            for (std::size_t i = 0; i < a.size(); ++i)
+                a[i] = Vector<U>{static_cast<U>(in[3 * i]), static_cast<U>(in[3 * i + 1]), static_cast<U>(in[3 * i + 2])};
+    }
+
     struct Key {
         std::size_t n = 0, s = 0;
         unsigned p = 0;

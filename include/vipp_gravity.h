@@ -103,7 +103,9 @@ int vipp_grav_export_expansions(vipp_grav_ctx *ctx, double *M, double *L);
  * entry of every cell at level >= 2, as the reference evaluates them),
  * [2] cells, [3] non-empty leaves, [4] depth, [5] M2L pairs whose source and
  * target both hold particles (the pairs that carry work), [6] leaves that took
- * the learned closure's fallback. */
+ * the learned closure's fallback, [7] interaction-list entries stored (every
+ * cell's AP and NB entries and its own NI entries, the inherited ones not
+ * repeated). */
 int vipp_grav_counts(vipp_grav_ctx *ctx, double *out, int32_t n);
 
 /* Per-phase device milliseconds accumulated since the last reset, measured with

@@ -100,7 +100,7 @@ class Oracle:
             self._rk4_direct = fn("rk4_direct", C.c_int, [_d, _d, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double])
             self._direct = fn("direct_accel", None, [_d, C.c_int64, C.c_double, C.c_double, C.c_double, _d])
             self._time_sample = fn("time_sample", C.c_int, [_d, C.c_int64, C.c_int64, C.c_uint32, C.c_double, C.c_double,
-                                                            C.c_double, C.c_int64, _d, _d])
+                                                            C.c_double, C.c_int64, C.c_int32, _d, _d])
         self._P = fn("P", None, [C.c_uint32, C.c_double, _d])
         self._DP = fn("DP", None, [C.c_uint32, C.c_double, _d])
         self._Y = fn("Y", None, [C.c_uint32, C.c_double, C.c_double, _d])
@@ -229,13 +229,16 @@ class Oracle:
         self._fit(e_t, t, len(t), C.byref(a), C.byref(c))
         return a.value, c.value
 
-    def time_sample(self, x, s, p, m, G, eps, per_phase):
-        """Reference FMM cost of one force evaluation, sampled: (seconds per phase, sampled fraction per phase)."""
+    def time_sample(self, x, s, p, m, G, eps, per_phase, threads=1):
+        """Reference FMM cost of one force evaluation, sampled: (seconds per
+        phase, sampled fraction per phase).  P2M, M2M, P2P and L2L/L2P run
+        per_phase * threads items on `threads` threads, M2L per_phase items on
+        one (the reference's M2L loop is not race-free)."""
         assert self.kind == "ref"
         x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
         secs = np.zeros(7)
         frac = np.zeros(7)
-        self._check(self._time_sample(x, len(x) // 3, s, p, m, G, eps, per_phase, secs, frac))
+        self._check(self._time_sample(x, len(x) // 3, s, p, m, G, eps, per_phase, int(threads), secs, frac))
         return secs, frac
 
 

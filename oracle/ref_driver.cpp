@@ -319,10 +319,19 @@ void ref_curve_fit(const double *e_t, const double *t, int64_t k, double *alpha,
 // ratio of work units (particles for P2M/L2P, cells for M2M/L2L, AP entries for
 // M2L, particle pairs for P2P).  Expansion storage is sized up front (zeros;
 // the operators' cost does not depend on the values).
-//   out[0..6]  extrapolated seconds: tree, link, P2M, M2M, M2L, P2P, L2L+L2P
+// Threads follow the reference's own OpenMP structure: the Octree ctor is
+// serial (octree.hpp:79-191), calc_link forks at its root (solver_ref.hpp:155-
+// 161), and the sampled items of P2M / M2M (solver_ref.hpp:192), P2P
+// (solver_ref.hpp:203-212) and L2L / L2P (solver_ref.hpp:225) run on `threads`
+// threads, each item writing only its own cell or points.  M2L runs on one
+// thread: the reference's flat loop copies each source Cell (fmm.hpp:281)
+// while other threads resize and write that cell's local expansion
+// (fmm.hpp:279,296), a data race; its time is reported for one core.
+//   out[0..6]  extrapolated seconds: tree, link, P2M, M2M, M2L (one thread),
+//              P2P, L2L+L2P
 //   info[0..6] sampled fraction of each phase (1 for the full phases)
-int ref_time_sample(const double *x, int64_t n, int64_t s, uint32_t p, double m, double G, double eps, int64_t per_phase, double *out,
-                    double *info) {
+int ref_time_sample(const double *x, int64_t n, int64_t s, uint32_t p, double m, double G, double eps, int64_t per_phase, int32_t threads,
+                    double *out, double *info) {
     return guarded([&] {
         using clk = std::chrono::steady_clock;
         auto el = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); };
@@ -342,8 +351,10 @@ int ref_time_sample(const double *x, int64_t n, int64_t s, uint32_t p, double m,
         for (size_t c = 1; c < nc; ++c) level[c] = level[cells[c].parent().value()] + 1;
         for (auto &c : cells) FMMTest::presize(c, p);
         std::vector<Vector<D>> acc(xv.size(), Vector<D>{});
-        // select every k-th of `items` (weights w) up to per_phase items, run f, scale by weight ratio
-        auto sampled = [&](const std::vector<size_t> &items, const std::vector<double> &w, auto &&f, double &secs, double &frac) {
+        // select every k-th of `items` (weights w) up to `per` items, run f on
+        // `nth` threads, scale by weight ratio
+        auto sampled = [&](const std::vector<size_t> &items, const std::vector<double> &w, auto &&f, double &secs, double &frac, int64_t per,
+                           int nth) {
             double tot = 0, done = 0;
             for (double q : w) tot += q;
             if (items.empty() || tot == 0) {
@@ -351,16 +362,21 @@ int ref_time_sample(const double *x, int64_t n, int64_t s, uint32_t p, double m,
                 frac = 1;
                 return;
             }
-            const size_t stride = std::max<size_t>(1, items.size() / static_cast<size_t>(std::max<int64_t>(1, per_phase)));
-            auto a = clk::now();
+            const size_t stride = std::max<size_t>(1, items.size() / static_cast<size_t>(std::max<int64_t>(1, per)));
+            std::vector<size_t> pick;
             for (size_t k = 0; k < items.size(); k += stride) {
-                f(items[k]);
+                pick.push_back(items[k]);
                 done += w[k];
             }
+            auto a = clk::now();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nth)
+            for (size_t k = 0; k < pick.size(); ++k) f(pick[k]);
             auto b = clk::now();
             frac = done / tot;
             secs = el(a, b) / frac;
         };
+        const int nth = std::max(1, threads);
+        const int64_t per_par = per_phase * nth;  // the same sampled work per thread
         std::vector<size_t> leaves2, inner2, targets, leavesall, l2cells;
         std::vector<double> wl2, wi2, wt, wla, wl2c;
         auto subtree_count = [&](auto &&self, size_t c) -> double {
@@ -394,13 +410,13 @@ int ref_time_sample(const double *x, int64_t n, int64_t s, uint32_t p, double m,
                 wl2c.push_back(1);
             }
         }
-        sampled(leaves2, wl2, [&](size_t c) { FMMTest::p2m(cells[c], p, xv, m); }, out[2], info[2]);
-        sampled(inner2, wi2, [&](size_t c) { FMMTest::m2m(cells[c], p, cells); }, out[3], info[3]);
-        sampled(targets, wt, [&](size_t c) { FMMTest::m2l(cells[c], p, cells, ap[c]); }, out[4], info[4]);
-        sampled(leavesall, wla, [&](size_t c) { FMMTest::p2p(cells[c], xv, acc, cells, ni[c], G, m, eps); }, out[5], info[5]);
+        sampled(leaves2, wl2, [&](size_t c) { FMMTest::p2m(cells[c], p, xv, m); }, out[2], info[2], per_par, nth);
+        sampled(inner2, wi2, [&](size_t c) { FMMTest::m2m(cells[c], p, cells); }, out[3], info[3], per_par, nth);
+        sampled(targets, wt, [&](size_t c) { FMMTest::m2l(cells[c], p, cells, ap[c]); }, out[4], info[4], per_phase, 1);
+        sampled(leavesall, wla, [&](size_t c) { FMMTest::p2p(cells[c], xv, acc, cells, ni[c], G, m, eps); }, out[5], info[5], per_par, nth);
         double s1 = 0, f1 = 1, s2 = 0, f2 = 1;
-        sampled(l2cells, wl2c, [&](size_t c) { FMMTest::l2l(cells[c], p, cells); }, s1, f1);
-        sampled(leaves2, wl2, [&](size_t c) { FMMTest::l2p(cells[c], p, xv, acc); }, s2, f2);
+        sampled(l2cells, wl2c, [&](size_t c) { FMMTest::l2l(cells[c], p, cells); }, s1, f1, per_par, nth);
+        sampled(leaves2, wl2, [&](size_t c) { FMMTest::l2p(cells[c], p, xv, acc); }, s2, f2, per_par, nth);
         out[6] = s1 + s2;
         info[6] = std::min(f1, f2);
     });

@@ -89,6 +89,32 @@ int main() {
     solver.preprocess(prob, x, v);
     CHECK(solver.context() == first);
 
+    // the float type set (main.cpp:9-16): coordinates are converted to and
+    // from the ABI's doubles, the device computes in FP32
+    using F = float;
+    std::vector<gravity::Vector<F>> xf, vf;
+    for (std::size_t i = 0; i < n; ++i) {
+        xf.emplace_back(static_cast<F>(x[i].x()), static_cast<F>(x[i].y()), static_cast<F>(x[i].z()));
+        vf.emplace_back(static_cast<F>(v[i].x()), static_cast<F>(v[i].y()), static_cast<F>(v[i].z()));
+        xr[3 * i] = xf[i].x(), xr[3 * i + 1] = xf[i].y(), xr[3 * i + 2] = xf[i].z();
+        vr[3 * i] = vf[i].x(), vr[3 * i + 1] = vf[i].y(), vr[3 * i + 2] = vf[i].z();
+    }
+    const gravity::Problem<F, F, F, F, F, F> probf{n, 8, 5, 1e-3f};
+    gravity::GpuSolver<F, F, F, F, F, F> solverf;
+    solverf.preprocess(probf, xf, vf);
+    solverf.solve(probf, 2.0f, xf, vf);
+    solverf.postprocess(probf, xf, vf);
+    cfg.precision = VIPP_GRAV_F32, cfg.epsilon = static_cast<double>(1e-3f), cfg.mass = static_cast<double>(F{1} / static_cast<F>(n));
+    CHECK(vipp_grav_create(&cfg, &c) == VIPP_GRAV_OK);
+    CHECK(vipp_grav_upload(c, xr.data(), vr.data(), nullptr) == VIPP_GRAV_OK);
+    CHECK(vipp_grav_step(c, static_cast<double>(probf.delta_t() / 2.0f), 1) == VIPP_GRAV_OK);
+    CHECK(vipp_grav_download(c, xr.data(), vr.data()) == VIPP_GRAV_OK);
+    vipp_grav_destroy(c);
+    for (std::size_t i = 0; i < n; ++i) {
+        CHECK(xf[i].x() == static_cast<F>(xr[3 * i]) && xf[i].y() == static_cast<F>(xr[3 * i + 1]) && xf[i].z() == static_cast<F>(xr[3 * i + 2]));
+        CHECK(vf[i].x() == static_cast<F>(vr[3 * i]) && vf[i].y() == static_cast<F>(vr[3 * i + 1]) && vf[i].z() == static_cast<F>(vr[3 * i + 2]));
+    }
+
     // errors surface as std::runtime_error
     bool threw = false;
     try {

@@ -3,7 +3,9 @@
 test_reference_benchmark_hook: the reference's own validation hook
 (Benchmark::benchmark, benchmark.hpp:90-175) compiled from the reference's
 headers with gravity::Solver replaced by gravity::GpuSolver (oracle/_ref/dropin,
-built from oracle/dropin_main.cpp).  It compares one GPU RK4 step of
+built from oracle/dropin_main.cpp), over main.cpp's three type sets (double,
+float as built with GRAVITY_FLOAT, float with a double time type as built with
+GRAVITY_TIME_DOUBLE; main.cpp:9-32).  It compares one GPU RK4 step of
 delta_t / t, t = 1..16, with the reference's FP64 SolverRef on the CPU, fits
 E(T) = alpha T^c and must print the reference's pass verdict (alpha below
 1e-6 sqrt(N) or c below -4.99, benchmark.hpp:160-168).
@@ -18,10 +20,14 @@ CPU alone, 2 of 12 runs at 16 threads ended in a segfault or std::bad_alloc;
 test_reference_different_precision: the same driver in its "precision" mode
 does what the reference's main.cpp:33-36 does when built with
 ENABLE_DIFFERENT_PRECISION: Benchmark::benchmark_different_precision
-(benchmark.hpp:31-87) runs GpuSolver over one of main.cpp's type sets (double;
-float with a double time type) against GpuSolver<float, ...> as its reference,
-and must print the pass verdict of the same rule.  Both solvers run on the
-device; the reference's CPU solver is not involved.
+(benchmark.hpp:31-87) runs GpuSolver over one of main.cpp's type sets against
+a float reference solver, and must print the pass verdict of the same rule.
+The float reference is GpuSolver<float, ...> (what the alias swap makes of
+main.cpp:34) or the reference's own SolverRef<float, ...> on the CPU.
+
+test_reference_search_delta_t: oracle/_ref/dropin_sdt, the driver built as
+experiment/search_timestep.sh builds the reference, writes the thirty-line
+"1/t,E" CSV of the time-step search.
 
 test_reference_output_files: oracle/_ref/dropin_out, the same driver built
 with the reference's ENABLE_PRINT and GRAVITY_CHECK_TIME switches, writes the
@@ -44,15 +50,17 @@ pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DROPIN = os.path.join(ROOT, "oracle", "_ref", "dropin")
 DROPIN_OUT = os.path.join(ROOT, "oracle", "_ref", "dropin_out")
+DROPIN_SDT = os.path.join(ROOT, "oracle", "_ref", "dropin_sdt")
 CPP_TEST = os.path.join(ROOT, "vipp-nbody-bench_amd", "_build", "solver_gpu_test")
 
 
-@pytest.mark.parametrize("mode", ["f64", "f32", "split"])
-def test_reference_benchmark_hook(mode):
+@pytest.mark.parametrize("types,precision", [("double", "f64"), ("double", "f32"), ("double", "split"), ("float", "auto"),
+                                             ("float", "split"), ("time_double", "split")])
+def test_reference_benchmark_hook(types, precision):
     if not os.path.exists(DROPIN):
         pytest.skip("oracle/_ref/dropin is built only where the reference checkout exists")
     env = dict(os.environ, OMP_NUM_THREADS="1")  # the reference's M2L loop is not race-free (module docstring)
-    r = subprocess.run([DROPIN, "4096", "10", mode], capture_output=True, text=True, timeout=240, env=env)
+    r = subprocess.run([DROPIN, "4096", "10", "bench", types, precision], capture_output=True, text=True, timeout=240, env=env)
     print(r.stdout, r.stderr)
     assert r.returncode == 0, r.stderr
     m = re.search(r"α:(\S+) c:(\S+)", r.stdout)  # benchmark.hpp:161
@@ -60,16 +68,59 @@ def test_reference_benchmark_hook(mode):
     assert "solver has passed the benchmark test" in r.stdout, r.stdout
 
 
-@pytest.mark.parametrize("types,ref", [("double", "f32"), ("double", "split"), ("time_double", "split")])
+@pytest.mark.parametrize("types,ref", [("double", "f32"), ("double", "split"), ("time_double", "split"), ("float", "f32"),
+                                       ("double", "cpu"), ("float", "cpu"), ("time_double", "cpu")])
 def test_reference_different_precision(types, ref):
+    """ref f32 / split: the float reference is GpuSolver<float, ...>, as the
+    alias swap of solver.hpp makes it in main.cpp:34; ref cpu: it is the
+    reference's own SolverRef<float, ...> on the CPU, so the GPU solver is
+    judged against the reference's arithmetic."""
     if not os.path.exists(DROPIN):
         pytest.skip("oracle/_ref/dropin is built only where the reference checkout exists")
-    r = subprocess.run([DROPIN, "4096", "10", "precision", types, ref], capture_output=True, text=True, timeout=240)
+    env = dict(os.environ, OMP_NUM_THREADS="1")  # the reference's M2L loop is not race-free (module docstring)
+    r = subprocess.run([DROPIN, "4096", "10", "precision", types, ref], capture_output=True, text=True, timeout=240, env=env)
     print(r.stdout, r.stderr)
     assert r.returncode == 0, r.stderr
     m = re.search(r"α:(\S+) c:(\S+)", r.stdout)  # benchmark.hpp:75
     assert m and all(v not in ("nan", "-nan", "inf", "-inf") for v in m.groups()), r.stdout
     assert "solver has passed the benchmark test" in r.stdout, r.stdout
+
+
+@pytest.mark.parametrize("types,precision", [("double", "f64"), ("float", "auto")])
+def test_reference_search_delta_t(types, precision):
+    """oracle/_ref/dropin_sdt, built as experiment/search_timestep.sh builds
+    the reference (GRAVITY_SEARCH_DELTA_T, GRAVITY_SUBDIVIDE_REF_TIMESTEP=2)
+    and run with its arguments (s = 10, epsilon = 0): delta_t() = 1, thirty
+    single steps of 1 / t, t = 1.5^k, each against two reference steps of
+    half that size, and one stderr line "1/t,E" per step
+    (benchmark.hpp:105-106, 135-146, 154-156) for make_graph_delta_t.py.
+    While the step is large the difference is RK4's own error, which falls
+    with the step's fifth power until rounding takes over."""
+    if not os.path.exists(DROPIN_SDT):
+        pytest.skip("oracle/_ref/dropin_sdt is built only where the reference checkout exists")
+    import numpy as np
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    r = subprocess.run([DROPIN_SDT, "500", "10", "bench", types, precision, "0"], capture_output=True, text=True, timeout=240, env=env)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, r.stderr
+    assert re.search(r"α:(\S+) c:(\S+)", r.stdout), r.stdout
+    rows = np.array([[float(v) for v in l.split(",")] for l in r.stderr.splitlines() if l.count(",") == 1])
+    assert rows.shape == (30, 2), r.stderr
+    dt = 1.5 ** -np.arange(30)
+    assert np.allclose(rows[:, 0], dt, rtol=1e-5, atol=0)  # printed with 6 significant digits
+    e = rows[:, 1]
+    assert np.isfinite(e).all() and (e >= 0).all()
+    assert (np.diff(e[:9]) < 0).all(), e  # the step's own error falls while it dominates
+    if types == "double":
+        # RK4's fifth-order local error down to FP64 rounding
+        slope = np.polyfit(np.log(dt[8:19]), np.log(e[8:19]), 1)[0]
+        print(f"log E / log dt slope over rows 8..18: {slope:.2f}")
+        assert 4.0 < slope < 6.0, (slope, e)
+        assert e[22:].max() < 1e-12, e
+    else:
+        # float positions: the difference stops at their rounding
+        floor = np.median(e[15:])
+        assert 1e-5 < floor < 1e-3 and e[15:].max() < 2 * floor, e
 
 
 def test_reference_output_files(tmp_path):
@@ -78,7 +129,7 @@ def test_reference_output_files(tmp_path):
     import numpy as np
     n, s = 2048, 10
     env = dict(os.environ, OMP_NUM_THREADS="1")  # the reference's M2L loop is not race-free (module docstring)
-    r = subprocess.run([DROPIN_OUT, str(n), str(s), "f64"], capture_output=True, text=True, timeout=240, env=env, cwd=tmp_path)
+    r = subprocess.run([DROPIN_OUT, str(n), str(s), "bench", "double", "f64"], capture_output=True, text=True, timeout=240, env=env, cwd=tmp_path)
     print(r.stdout, r.stderr)
     assert r.returncode == 0, r.stderr
     m = re.search(r"α:(\S+) c:(\S+)", r.stdout)

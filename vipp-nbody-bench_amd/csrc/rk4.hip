@@ -82,13 +82,16 @@ __global__ __launch_bounds__(256) void k_energy(int64_t n, int kind, const doubl
 }
 
 // work counts: [0] P2P pairs (i != j), [1] M2L pairs of targets at level >= 2,
-// [2] M2L pairs whose target and source both hold particles
+// [2] M2L pairs whose target and source both hold particles, [3] list
+// entries stored (AP, NB and each cell's own NI entries)
 __global__ __launch_bounds__(256) void k_count(int64_t nc, const int8_t *__restrict__ level, const int32_t *__restrict__ child, const uint32_t *__restrict__ count,
                                                const int32_t *__restrict__ anc, const int32_t *__restrict__ ni, const int32_t *__restrict__ ni_cnt,
-                                               const int32_t *__restrict__ ap, const int32_t *__restrict__ ap_cnt, unsigned long long *__restrict__ out) {
+                                               const int32_t *__restrict__ ap, const int32_t *__restrict__ ap_cnt, const int32_t *__restrict__ nb_cnt,
+                                               unsigned long long *__restrict__ out) {
     const int64_t c = blockIdx.x * 256ll + threadIdx.x;
-    unsigned long long pairs = 0, m2l = 0, live = 0;
+    unsigned long long pairs = 0, m2l = 0, live = 0, entries = 0;
     if (c < nc) {
+        entries = static_cast<unsigned long long>(ap_cnt[c] + nb_cnt[c] + ni_cnt[c]);
         if (level[c] >= 2) {
             m2l = static_cast<unsigned long long>(ap_cnt[c]);
             if (count[c] > 0)
@@ -106,6 +109,7 @@ __global__ __launch_bounds__(256) void k_count(int64_t nc, const int8_t *__restr
     if (pairs) atomicAdd(&out[0], pairs);
     if (m2l) atomicAdd(&out[1], m2l);
     if (live) atomicAdd(&out[2], live);
+    if (entries) atomicAdd(&out[3], entries);
 }
 
 }  // namespace
@@ -164,11 +168,11 @@ double potential_energy(Ctx &c) { return -0.5 * c.G * reduce_energy(c, 1); }
 void count_work(Ctx &c) {
     TreeDev &t = c.t;
     DevBuf<unsigned long long> out;
-    out.ensure(3);
-    VG_HIP(hipMemsetAsync(out.p, 0, 3 * sizeof(unsigned long long), c.stream));
-    k_count<<<ceil_div(t.ncells, 256), 256, 0, c.stream>>>(t.ncells, t.level, t.child, t.count, t.anc, t.ni, t.ni_cnt, t.ap, t.ap_cnt, out);
+    out.ensure(4);
+    VG_HIP(hipMemsetAsync(out.p, 0, 4 * sizeof(unsigned long long), c.stream));
+    k_count<<<ceil_div(t.ncells, 256), 256, 0, c.stream>>>(t.ncells, t.level, t.child, t.count, t.anc, t.ni, t.ni_cnt, t.ap, t.ap_cnt, t.nb_cnt, out);
     VG_LAUNCH_CHECK();
-    unsigned long long h[3];
+    unsigned long long h[4];
     VG_HIP(hipMemcpyAsync(h, out.p, sizeof(h), hipMemcpyDeviceToHost, c.stream));
     VG_HIP(hipStreamSynchronize(c.stream));
     c.counts[0] = static_cast<double>(h[0]);
@@ -177,6 +181,7 @@ void count_work(Ctx &c) {
     c.counts[3] = static_cast<double>(t.nleaves);
     c.counts[4] = static_cast<double>(t.depth);
     c.counts[5] = static_cast<double>(h[2]);
+    c.counts[7] = static_cast<double>(h[3]);
 }
 
 }  // namespace vg

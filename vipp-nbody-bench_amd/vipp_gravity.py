@@ -273,7 +273,7 @@ class Gravity:
         out = np.zeros(8)
         self._check(self.lib.vipp_grav_counts(self.h, _ptr(out), 8))
         return {"p2p_pairs": out[0], "m2l_pairs": out[1], "cells": int(out[2]), "leaves": int(out[3]), "depth": int(out[4]),
-                "m2l_live_pairs": out[5], "fallback_leaves": int(out[6])}
+                "m2l_live_pairs": out[5], "fallback_leaves": int(out[6]), "list_entries": out[7]}
 
     def set_timing(self, on=True):
         self._check(self.lib.vipp_grav_set_timing(self.h, 1 if on else 0))

@@ -83,13 +83,18 @@ def two_plummer(n, seed=0, sep=5.0, vbulk=0.3):
     return np.concatenate([x1, x2]), np.concatenate([v1, v2])
 
 
-# The five workloads: (generator, N, s, p, eps, dt, steps)
+# The workloads: N, s, p, softening, dt, steps of the configuration and the
+# arithmetic it names (f64, f32 = plain FP32, split = BF16 hi/lo on MFMA)
 WORKLOADS = {
-    "plummer16k_p4": dict(n=16384, s=64, p=4, eps=1e-3, dt=1e-3, steps=10),
-    "uniform1m_p8": dict(n=1 << 20, s=64, p=8, eps=1e-4, dt=1e-4, steps=10),
-    "plummer4m_p12": dict(n=1 << 22, s=64, p=12, eps=1e-3, dt=5e-4, steps=20),
-    "mixture4m_p4": dict(n=1 << 22, s=64, p=4, eps=1e-3, dt=5e-4, steps=20),
-    "collision16m": dict(n=1 << 24, s=64, p=12, eps=1e-3, dt=1e-3, steps=100),
+    "plummer16k_p4": dict(n=16384, s=64, p=4, eps=1e-3, dt=1e-3, steps=10, mode="f64"),
+    "uniform1m_p8": dict(n=1 << 20, s=64, p=8, eps=1e-4, dt=1e-4, steps=10, mode="f32"),
+    "plummer4m_p12": dict(n=1 << 22, s=64, p=12, eps=1e-3, dt=5e-4, steps=20, mode="split"),
+    "mixture4m_p4": dict(n=1 << 22, s=64, p=4, eps=1e-3, dt=5e-4, steps=20, mode="split"),
+    "collision16m": dict(n=1 << 24, s=64, p=12, eps=1e-3, dt=1e-3, steps=100, mode="split"),
+    # the reference's own initial conditions (Problem::initialize, problem.hpp:188-211,
+    # mt19937_64 seed 0, the mass fraction truncated at 1 - eps) at the headline's size
+    "refinit4m_p12": dict(n=1 << 22, s=64, p=12, eps=1e-3, dt=5e-4, steps=3, mode="split",
+                          source="the reference's Problem::initialize stream (vipp_grav_problem_initialize)"),
 }
 
 
@@ -107,5 +112,9 @@ def make(name, seed=0):
         return x, v, None
     if name.startswith("collision"):
         x, v = two_plummer(n, seed)
+        return x, v, None
+    if name.startswith("refinit"):
+        import vipp_gravity
+        x, v, _ = vipp_gravity.problem_initialize(n, seed, w["eps"])
         return x, v, None
     raise KeyError(name)
