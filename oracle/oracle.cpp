@@ -710,7 +710,7 @@ void orc_tree_export(const orc_tree *t, int64_t *parent, int32_t *level, double 
 }
 
 // Manual subdivision of one leaf, as the reference's tests do through the
-// friend hook (test_fmm.cpp:254-256 calling octree.hpp:166-191).
+// friend hook (test_fmm.cpp:97-99 calling octree.hpp:166-191; used at test_fmm.cpp:346,364-365).
 int orc_tree_subdivide(orc_tree *t, int64_t idx, int64_t threshold) {
     return guarded([&] {
         if (idx < 0 || static_cast<size_t>(idx) >= t->cells.size() || !t->cells[static_cast<size_t>(idx)].leaf)
@@ -769,7 +769,7 @@ int orc_tree_fmm(orc_tree *t, const double *x, const double *masses, double m, d
 
 // Upward pass (P2M on leaves, M2M on internal cells) for every cell at level
 // >= min_level; the reference gates it at level 2 (solver_ref.hpp:187,196),
-// its tests call p2m/m2m on the root directly (test_fmm.cpp:483-512).
+// its tests call p2m/m2m on the root directly (test_fmm.cpp:326-355).
 int orc_tree_upward(orc_tree *t, const double *masses, double m, uint32_t p, int32_t min_level, double *Mout) {
     return guarded([&] {
         Expansions E(t->cells.size(), p);

@@ -162,7 +162,7 @@ def test_closure_gain_over_plain_low_order(vg, cl, orc, trained):
     e4, ec = _rel(a4, a12), _rel(ac, a12)
     print(f"median rel. difference to p=12: p=4 {np.median(e4):.3e}, p=4 + closure {np.median(ec):.3e}")
     assert cnt["fallback_leaves"] == 0
-    assert np.median(ec) < 0.95 * np.median(e4)
+    assert np.median(ec) < 0.90 * np.median(e4)  # measured: -14%
 
 
 def test_closure_fallback(vg, cl, orc, trained):
@@ -186,3 +186,39 @@ def test_closure_fallback(vg, cl, orc, trained):
     e4, ep = _rel(a4, a12), _rel(ap, a12)
     print(f"bound {b:.3e}: {cntp['fallback_leaves']} of {far_leaves} leaves fall back; median to p=12 {np.median(ep):.3e} (p=4 {np.median(e4):.3e})")
     assert np.median(ep) < np.median(e4)
+
+
+def test_closure_pipeline_on_mixture(vg, cl):
+    """The train / evaluate / fallback pipeline on the clustered workload: a
+    2^20-particle mixture of 32 Plummer spheres (softening 1e-4), trained on
+    two seeds, evaluated on a third.  The network must not make the p = 4
+    far field worse against the order-12 FMM (on the mixture its inputs
+    predict little: the measured gain is a few percent); bound 0 must give the
+    order-12 far field; a bound for a quarter of the training leaves must send
+    between 10% and 50% of the leaves to order 12 and land between the two."""
+    import workloads
+    eps, n = 1e-4, 1 << 20
+    X, T = [], []
+    for seed in (21, 22):
+        xs, _ = workloads.plummer_mixture(n, seed=seed)
+        a, b = cl.training_pairs(xs, 64, eps)
+        X.append(a)
+        T.append(b)
+    X, T = np.concatenate(X), np.concatenate(T)
+    params, io, hist = cl.train(X, T, epochs=20, batch=512, lr=2e-3, seed=0)
+    assert hist[-1] < hist[0]
+    x, _ = workloads.plummer_mixture(n, seed=23)
+    a12, _ = _forces(vg, x, 12, eps)
+    a4, _ = _forces(vg, x, 4, eps)
+    ac, cnt = _forces(vg, x, 4, eps, params, io)
+    e4, ec = np.median(_rel(a4, a12)), np.median(_rel(ac, a12))
+    assert cnt["fallback_leaves"] == 0 and ec <= e4, (e4, ec)
+    af, cntf = _forces(vg, x, 4, eps, params, io, bound=0.0)
+    assert np.median(_rel(af, a12)) < 1e-6
+    b = cl.bound_for_fraction(params, io, X, 0.25)
+    ap, cntp = _forces(vg, x, 4, eps, params, io, bound=b)
+    leaves = cntf["fallback_leaves"]
+    assert 0.1 * leaves < cntp["fallback_leaves"] < 0.5 * leaves, (cntp["fallback_leaves"], leaves)
+    ep = np.median(_rel(ap, a12))
+    print(f"mixture: median to p=12: p=4 {e4:.3e}, closure {ec:.3e}, 25% fallback {ep:.3e}")
+    assert np.median(_rel(af, a12)) < ep < e4

@@ -62,7 +62,7 @@ def test_four_point_tree(orc):  # test_octree.cpp:130-154
 
 
 # ------------------------------------------------------ special functions ---
-def test_legendre_at_minus_one(orc):  # test_fmm.cpp:288-316
+def test_legendre_at_minus_one(orc):  # test_fmm.cpp:131-159
     P = orc.P(5, -1.0)
     for n in range(6):
         assert abs(P[n, 0] - (-1) ** n) <= TOL
@@ -70,7 +70,7 @@ def test_legendre_at_minus_one(orc):  # test_fmm.cpp:288-316
             assert abs(P[n, m]) <= TOL
 
 
-def test_legendre_at_zero(orc):  # test_fmm.cpp:318-346
+def test_legendre_at_zero(orc):  # test_fmm.cpp:161-189
     P = orc.P(5, 0.0)
     exp = {(0, 0): 1, (1, 0): 0, (2, 0): -0.5, (3, 0): 0, (4, 0): 3 / 8, (5, 0): 0, (1, 1): -1, (2, 1): 0, (3, 1): 1.5,
            (4, 1): 0, (5, 1): -15 / 8, (2, 2): 3, (3, 2): 0, (4, 2): -7.5, (5, 2): 0, (3, 3): -15, (4, 3): 0,
@@ -79,7 +79,7 @@ def test_legendre_at_zero(orc):  # test_fmm.cpp:318-346
         assert abs(P[n, m] - v) <= TOL, (n, m, P[n, m], v)
 
 
-def test_legendre_at_one(orc):  # test_fmm.cpp:348-376
+def test_legendre_at_one(orc):  # test_fmm.cpp:191-219
     P = orc.P(5, 1.0)
     for n in range(6):
         assert abs(P[n, 0] - 1) <= TOL
@@ -87,7 +87,7 @@ def test_legendre_at_one(orc):  # test_fmm.cpp:348-376
             assert abs(P[n, m]) <= TOL
 
 
-def test_y_at_pole(orc):  # test_fmm.cpp:378-389
+def test_y_at_pole(orc):  # test_fmm.cpp:221-232
     Y = orc.Y(5, 0.0, 0.0)
     for k in range(5):
         for m in range(-k, k):
@@ -95,7 +95,7 @@ def test_y_at_pole(orc):  # test_fmm.cpp:378-389
             assert abs(Y[k, 5 + m] - exp) <= TOL
 
 
-def test_y_quarter_pi(orc):  # test_fmm.cpp:391-435 (phi = 0, so every value is real)
+def test_y_quarter_pi(orc):  # test_fmm.cpp:234-278 (phi = 0, so every value is real)
     Y = orc.Y(5, math.pi / 4, 0.0)
     s2 = math.sqrt(2)
     half = {
@@ -115,7 +115,7 @@ def test_y_quarter_pi(orc):  # test_fmm.cpp:391-435 (phi = 0, so every value is 
             assert abs(Y[k, 5 + mm] - v) <= TOL, (k, mm)
 
 
-def test_y_half_pi(orc):  # test_fmm.cpp:437-481
+def test_y_half_pi(orc):  # test_fmm.cpp:280-324
     Y = orc.Y(5, math.pi / 2, 0.0)
     half = {(0, 0): 1, (1, 0): 0, (1, 1): -1 / math.sqrt(2), (2, 0): -0.5, (2, 1): 0, (2, 2): 3 / math.sqrt(24),
             (3, 0): 0, (3, 1): 1.5 / math.sqrt(12), (3, 2): 0, (3, 3): -15 / math.sqrt(720),
@@ -128,14 +128,14 @@ def test_y_half_pi(orc):  # test_fmm.cpp:437-481
 
 
 # ----------------------------------------------------------- operators ------
-def test_p2m_at_centre(orc):  # test_fmm.cpp:483-495
+def test_p2m_at_centre(orc):  # test_fmm.cpp:326-338
     x = np.array([[1.0, 1.0, 1.0]])
     with orc.build(x, 1) as h:
         M = h.upward(2, 1.0, min_level=0)
     assert abs(M[0, 0, 2] - 1) <= TOL and abs(M[0, 1, 2]) <= TOL and abs(M[0, 1, 3]) <= TOL
 
 
-def test_m2m_monopole_dipole(orc):  # test_fmm.cpp:497-512
+def test_m2m_monopole_dipole(orc):  # test_fmm.cpp:340-355
     x = np.array([[1.0, 0.0, 0.0]])
     with orc.build(x, 1) as h:
         h.subdivide(0, 2)
@@ -143,7 +143,7 @@ def test_m2m_monopole_dipole(orc):  # test_fmm.cpp:497-512
     assert abs(M[0, 0, 2] - 1) <= TOL and abs(M[0, 1, 2]) <= TOL
 
 
-def test_m2l_closed_form(orc):  # test_fmm.cpp:514-544: L_{1,1} of cell 9 at p=2
+def test_m2l_closed_form(orc):  # test_fmm.cpp:357-387: L_{1,1} of cell 9 at p=2
     x = np.array([[1, 1, 1], [1, -1, -1], [-1, -1, -1]], float)
     p = 2
     with orc.build(x, 1) as h:
@@ -171,7 +171,7 @@ def test_m2l_closed_form(orc):  # test_fmm.cpp:514-544: L_{1,1} of cell 9 at p=2
     assert abs(got.real - exp.real) <= TOL and abs(got.imag - exp.imag) <= TOL, (got, exp)
 
 
-def test_many_points_finite(orc):  # test_fmm.cpp:259-286 (N=1000 Problem, s=5, p=5: no NaN)
+def test_many_points_finite(orc):  # test_fmm.cpp:102-129 (N=1000 Problem, s=5, p=5: no NaN)
     x, v, e = orc.problem_initialize(1000, 0, None)
     out = orc.fmm(x, 5, 5, eps=e, diagnostics=True)
     assert np.isfinite(out["M"]).all() and np.isfinite(out["L"]).all() and np.isfinite(out["a"]).all()
