@@ -123,6 +123,19 @@ int vipp_grav_set_timing(vipp_grav_ctx *ctx, int32_t enabled);
  * device; meant for force-error samples of large N. */
 int vipp_grav_direct(vipp_grav_ctx *ctx, const int64_t *idx, int64_t nidx, double *a, double *pot);
 
+/* vipp_grav_direct with idx (nidx int64), a (3 nidx doubles) and pot (nidx
+ * doubles, may be NULL) in device memory of the context's device, for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation); no counterpart in the reference.  It
+ * allocates nothing: its partial sums live in a static device array of the
+ * library (16 MiB per device, held until the process ends), so nidx is at
+ * most 524288 (E_INVALID above).  Results agree with vipp_grav_direct to FP64
+ * rounding (the source range may be cut into fewer slices).  Calls on one
+ * device share that array: calls that overlap on one device race.  An index
+ * outside [0, N) gives E_INVALID after the call (outputs unspecified).
+ * Returns after the device work has finished. */
+int vipp_grav_direct_noalloc(vipp_grav_ctx *ctx, const int64_t *idx, int64_t nidx, double *a, double *pot);
+
 /* ---- Learned closure -------------------------------------------------------
  * A low-order FMM (p >= 4, FP32 expansions) whose leaf local expansions are
  * corrected before L2P by a small network: 28 inputs (the leaf's first 25

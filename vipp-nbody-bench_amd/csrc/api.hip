@@ -397,6 +397,7 @@ int vipp_grav_create(const vipp_grav_config *cfg, vipp_grav_ctx **out) {
             VG_HIP(hipHostMalloc(&c.pinned, vg::Ctx::PINNED_BYTES, hipHostMallocDefault));
             for (auto &e : c.ev) VG_HIP(hipEventCreate(&e));
             vg::fmm_constants_init();
+            vg::noalloc_init(c.device);
             const size_t n3 = 3 * static_cast<size_t>(c.n);
             c.x.ensure(n3);
             c.v.ensure(n3);
@@ -559,6 +560,17 @@ int vipp_grav_direct(vipp_grav_ctx *ctx, const int64_t *idx, int64_t nidx, doubl
         need(c.uploaded, VIPP_GRAV_E_STATE, "direct before upload");
         VG_HIP(hipSetDevice(c.device));
         vg::direct_subset(c, idx, nidx, a, pot);
+    });
+}
+
+int vipp_grav_direct_noalloc(vipp_grav_ctx *ctx, const int64_t *idx, int64_t nidx, double *a, double *pot) {
+    return guarded(ctx, [&] {
+        need(ctx != nullptr, VIPP_GRAV_E_INVALID, "null context");
+        need(nidx >= 0 && (nidx == 0 || (idx && a)), VIPP_GRAV_E_INVALID, "null argument");
+        vg::Ctx &c = ctx->c;
+        need(c.uploaded, VIPP_GRAV_E_STATE, "direct before upload");
+        VG_HIP(hipSetDevice(c.device));
+        vg::direct_subset_noalloc(c, idx, nidx, a, pot);
     });
 }
 

@@ -190,6 +190,11 @@ void l2l_list(Ctx &c, const int32_t *parents, int64_t np);  // L2L into the chil
 void ml_split_level(Ctx &c, const int32_t *ids, int64_t n, bool m2m);  // M2M (m2m) or L2L of one level on the matrix cores
 void l2p_list(Ctx &c, const int32_t *leaves, int64_t n, const uint8_t *skip);  // L2P of the given leaves (FP32)
 void direct_subset(Ctx &c, const int64_t *idx, int64_t nidx, double *a, double *pot);  // FP64 O(N nidx) oracle forces
+// the same with idx, a and pot in device memory, its partial sums in the
+// library's static device scratch (no allocation); noalloc_init resolves that
+// scratch's address on a device (once per device, with the device current)
+void direct_subset_noalloc(Ctx &c, const int64_t *idx, int64_t nidx, double *a, double *pot);
+void noalloc_init(int device);
 
 void phase_begin(Ctx &c, Phase p);
 void phase_end(Ctx &c, Phase p);
