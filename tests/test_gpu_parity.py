@@ -186,6 +186,43 @@ def test_fmm_f64_vs_oracle_masses_and_G(vg, orc):
         assert np.abs(a - ref).max() <= 1e-11 * np.abs(ref).max()
 
 
+def test_masses_and_G_closed_forms(vg):
+    """The device's mass and G handling against what the mathematics fixes,
+    independently of the oracle: the two-body direct sum and potential with
+    distinct masses and G != 1; Newton's third law of the direct sum; the FMM
+    exactly linear in G with G on the far field, affine with the reference's
+    far field (G only in P2P, fmm.hpp:539), and equal (to rounding) on a
+    uniform mass array and the scalar-mass path."""
+    G, eps, m0, m1 = 2.5, 0.1, 0.3, 1.7
+    x = np.array([[0.1, -0.2, 0.3], [0.45, 0.05, -0.15]])
+    d = x[1] - x[0]
+    r2 = d @ d + eps * eps
+    with vg.Gravity(2, 1, 4, "f64", eps=eps, G=G) as gr:
+        gr.upload(x, masses=np.array([m0, m1]))
+        a, pot = gr.direct([0, 1], potential=True)
+    assert np.allclose(a, [G * m1 * d / r2 ** 1.5, -G * m0 * d / r2 ** 1.5], rtol=1e-14, atol=0)
+    assert np.allclose(pot, [m1 / np.sqrt(r2), m0 / np.sqrt(r2)], rtol=1e-14, atol=0)
+    rng = np.random.default_rng(11)
+    x = rng.random((4000, 3))
+    m = rng.uniform(0.1, 3.0, len(x)) / len(x)
+    with vg.Gravity(len(x), 10, 8, "f64", eps=1e-3, G=1.9) as gr:
+        gr.upload(x, masses=m)
+        a = gr.direct(np.arange(len(x)))
+    assert np.abs((m[:, None] * a).sum(0)).max() <= 1e-12 * (m[:, None] * np.abs(a)).sum()
+
+    def fmm(G, ref_far_g, masses=m, mass=None):
+        with vg.Gravity(len(x), 10, 8, "f64", eps=1e-3, G=G, mass=mass, ref_far_g=ref_far_g) as gr:
+            gr.upload(x, masses=masses)
+            return gr.accelerations()
+    one = fmm(1.0, False)
+    assert np.abs(fmm(2.5, False) - 2.5 * one).max() <= 1e-14 * 2.5 * np.abs(one).max()
+    far = fmm(0.0, True)  # the reference's far field alone
+    assert np.abs(fmm(2.5, True) - (far + 2.5 * (one - far))).max() <= 1e-13 * np.abs(one).max()
+    # the scalar-mass path factors the mass out of the pair sums: equal to rounding
+    u = fmm(1.0, False, masses=np.full(len(x), 1.0 / len(x)))
+    assert np.abs(u - fmm(1.0, False, masses=None, mass=1.0 / len(x))).max() <= 1e-13 * np.abs(u).max()
+
+
 def test_rk4_step_f64_golden(vg, golden):
     g = golden("rk4_1k")
     x, v = g["x"], g["v"]

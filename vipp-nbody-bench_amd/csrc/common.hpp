@@ -74,4 +74,42 @@ struct DevBuf {
 __host__ __device__ inline int kidx_re(int n, int m) { return m == 0 ? n * n : n * n + 2 * m - 1; }
 __host__ __device__ inline int kidx_im(int n, int m) { return n * n + 2 * m; }
 
+// M2L by mirror symmetry.  The 189 well-separated (parent neighbour d, source
+// octant q) pairs of a target of octant 0 are walked in one fixed order; for a
+// target of octant qt the geometry is octant 0's mirrored in the axes where qt
+// has a bit set (d -> mirror(d), q -> q ^ qt), so the cell offset, and with it
+// the operator T_e, is octant 0's.  m2l_walk fills, per step of that order,
+// the operator index o (offset + (3,3,3) in base 7), the parent-neighbour slot
+// dn (0..26) of the mirrored neighbour and the mirrored source octant qs.
+__device__ inline void m2l_walk(int qt, int16_t *o, int16_t *dn, int16_t *qs) {
+    int st = 0;
+    for (int d = 0; d < 27; ++d) {
+        const int dx = d % 3 - 1, dy = (d / 3) % 3 - 1, dz = d / 9 - 1;
+        for (int q = 0; q < 8; ++q) {
+            const int ex = 2 * dx - (q & 1), ey = 2 * dy - ((q >> 1) & 1), ez = 2 * dz - ((q >> 2) & 1);
+            if (ex >= -1 && ex <= 1 && ey >= -1 && ey <= 1 && ez >= -1 && ez <= 1) continue;  // adjacent: near field
+            const int mx = (qt & 1) ? -dx : dx, my = (qt & 2) ? -dy : dy, mz = (qt & 4) ? -dz : dz;
+            o[st] = static_cast<int16_t>((ex + 3) + 7 * (ey + 3) + 49 * (ez + 3));
+            dn[st] = static_cast<int16_t>((mx + 1) + 3 * (my + 1) + 9 * (mz + 1));
+            qs[st] = static_cast<int16_t>(q ^ qt);
+            ++st;
+        }
+    }
+}
+
+// The mirrored operator is D_qt T_e D_qt with D_qt diagonal +-1; its entry for
+// real-packed coefficient k (x -> -x: (-1)^m on cosine, -(-1)^m on sine
+// coefficients; y -> -y: -1 on sine; z -> -z: (-1)^(n+m)).
+__host__ __device__ inline int m2l_mirror_sign(int qt, int k) {
+    int nn = 0;  // k = n^2 + r: degree n, r = 0 (m = 0), 2m - 1 (cosine) or 2m (sine)
+    while ((nn + 1) * (nn + 1) <= k) ++nn;
+    const int r = k - nn * nn, m = (r + 1) >> 1;
+    const bool im = r > 0 && (r & 1) == 0;
+    int sg = 1;
+    if (qt & 1) sg *= ((m & 1) ? -1 : 1) * (im ? -1 : 1);
+    if ((qt & 2) && im) sg = -sg;
+    if ((qt & 4) && ((nn + m) & 1)) sg = -sg;
+    return sg;
+}
+
 }  // namespace vg

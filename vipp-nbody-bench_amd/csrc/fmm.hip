@@ -410,94 +410,168 @@ __global__ __launch_bounds__(256) void k_l2l(const int32_t *__restrict__ ids, in
 
 
 // ---------------------------------------------------------------------------
-// Plain M2L: targets are the children of octant qt of a block of internal
-// parents; for each of the 27 parent-neighbour offsets and 8 source octants
-// the (static) cell offset selects one operator, applied as a K x K by K x TB
+// Plain M2L: targets are the children of octant qt of a chunk of TB internal
+// parents; for each well-separated (parent neighbour, source octant) pair the
+// (static) cell offset selects one operator, applied as a K x K by K x TB
 // product with the gathered source multipoles (offset-grouped GEMM on VALU).
+// As on the matrix cores, every block walks octant 0's 189 offsets (m2l_walk)
+// and the mirror signs D_qt go on the gathered multipoles and on the result,
+// so all blocks share 189 operators.  Blocks are placed XCD-aware: the eight
+// octant blocks of a parent chunk run side by side on one XCD and share its
+// L2.  The prologue tabulates, per parent neighbour and target, the first
+// child and the mask of non-empty children, and from them the offsets that
+// hold any source; only those are walked.  A thread owns RPT rows
+// (tr + 32 i) and CPT consecutive targets; the operator streams through LDS
+// in chunks of KC columns, double-buffered through registers (one barrier per
+// chunk), and the next offset's first chunk is fetched during the last.
 // ---------------------------------------------------------------------------
-template <class R, int TB>
+template <class R, int TB, int RPT>
 __global__ __launch_bounds__(256) void k_m2l_plain(const int32_t *__restrict__ parents, int nparents, const int32_t *__restrict__ child,
                                                    const uint32_t *__restrict__ count, const int32_t *__restrict__ nbr27, const R *__restrict__ Mx,
                                                    R *__restrict__ Lx, const R *__restrict__ op, int K, int ldk) {
-    constexpr int KC = 16;
-    constexpr int CPT = TB / 8;  // targets per thread
-    constexpr int SMS = TB + 1, SOS = 193;  // padded strides: no bank conflicts
-    __shared__ R sM[MAXK * SMS];
-    __shared__ R sOp[KC * SOS];
-    __shared__ int32_t par[TB], tgt[TB], src[TB];
-    __shared__ int any;
-    const int qt = blockIdx.y;
-    const int tr = threadIdx.x & 31, tc = threadIdx.x >> 5;
-    if (threadIdx.x < TB) {
-        const int pi = blockIdx.x * TB + threadIdx.x;
-        int32_t P = pi < nparents ? parents[pi] : -1;
+    constexpr int KC = sizeof(R) == 4 ? 16 : 8;     // operator columns per chunk
+    constexpr int CPT = TB / 8;                      // targets per thread
+    constexpr int VW = 16 / sizeof(R);               // elements per 16-byte LDS read
+    constexpr int SMS = TB + VW;                     // multipole column stride (16-byte rows)
+    constexpr int SOS = 32 * RPT + 1;                // operator column stride (odd: no bank conflicts)
+    constexpr int NPF = KC * 32 * RPT / 256;         // prefetched operator words per thread
+    static_assert(CPT % VW == 0 && (KC * 32 * RPT) % 256 == 0, "tile shape");
+    extern __shared__ __align__(16) unsigned char m2l_smem[];
+    R *sM = reinterpret_cast<R *>(m2l_smem);         // K x SMS: column k of the gathered, signed multipoles
+    R *sOp = sM + K * SMS;                           // 2 x KC x SOS: operator columns, transposed
+    __shared__ R sg[32 * RPT];                       // D_qt
+    __shared__ int32_t s_nc[27 * TB];                // first child of parent neighbour d of target t, or -1
+    __shared__ uint8_t s_nm[27 * TB];                // its non-empty children
+    __shared__ int32_t tgt[TB];
+    __shared__ int16_t s_o[189], s_dn[189], s_qs[189], s_live[189];
+    __shared__ int s_nlive;
+    const int nchunks = (nparents + TB - 1) / TB;
+    const int w = (blockIdx.x & 7) * nchunks + (blockIdx.x >> 3);  // XCD x takes work items [x W / 8, (x + 1) W / 8)
+    if (w >= 8 * nchunks) return;
+    const int chunk = w >> 3, qt = w & 7;
+    const int tid = threadIdx.x, tr = tid & 31, tc = tid >> 5;
+    if (tid == 0) m2l_walk(qt, s_o, s_dn, s_qs);
+    for (int k = tid; k < 32 * RPT; k += 256) sg[k] = static_cast<R>(k < K ? m2l_mirror_sign(qt, k) : 0);
+    if (tid < TB) {
+        const int pi = chunk * TB + tid;
+        const int32_t P = pi < nparents ? parents[pi] : -1;
         int32_t t = P >= 0 ? child[P] + qt : -1;
         if (t >= 0 && count[t] == 0) t = -1;
-        par[threadIdx.x] = t >= 0 ? P : -1;
-        tgt[threadIdx.x] = t;
-    }
-    R acc[6][CPT];
-    for (int i = 0; i < 6; ++i)
-        for (int jj = 0; jj < CPT; ++jj) acc[i][jj] = 0;
-    const int tx = 1 - (qt & 1), ty = 1 - ((qt >> 1) & 1), tz = 1 - ((qt >> 2) & 1);
-    for (int dn = 0; dn < 27; ++dn) {
-        const int dx = dn % 3 - 1, dy = (dn / 3) % 3 - 1, dz = dn / 9 - 1;
-        for (int qs = 0; qs < 8; ++qs) {
-            const int ex = 2 * dx + (1 - (qs & 1)) - tx, ey = 2 * dy + (1 - ((qs >> 1) & 1)) - ty, ez = 2 * dz + (1 - ((qs >> 2) & 1)) - tz;
-            if (ex >= -1 && ex <= 1 && ey >= -1 && ey <= 1 && ez >= -1 && ez <= 1) continue;  // adjacent: P2P
-            const int o = (ex + 3) + 7 * (ey + 3) + 49 * (ez + 3);
-            __syncthreads();
-            if (threadIdx.x == 0) any = 0;
-            __syncthreads();
-            if (threadIdx.x < TB) {
-                const int32_t P = par[threadIdx.x];
-                int32_t s = -1;
-                if (P >= 0) {
-                    const int32_t sp = nbr27[27 * static_cast<int64_t>(P) + dn];
-                    if (sp >= 0) {
-                        s = child[sp] + qs;
-                        if (count[s] == 0) s = -1;
-                    }
-                }
-                src[threadIdx.x] = s;
-                if (s >= 0) any = 1;
-            }
-            __syncthreads();
-            if (!any) continue;
-            for (int i = threadIdx.x; i < K * TB; i += 256) {
-                const int t = i / K, col = i % K;
-                const int32_t s = src[t];
-                sM[col * SMS + t] = s >= 0 ? Mx[static_cast<size_t>(s) * ldk + col] : R(0);
-            }
-            const R *O = op + static_cast<size_t>(16 + o) * K * K;
-            for (int c0 = 0; c0 < K; c0 += KC) {
-                __syncthreads();
-                for (int i = threadIdx.x; i < KC * K; i += 256) {
-                    const int r = i / KC, cc = i % KC;
-                    sOp[cc * SOS + r] = (c0 + cc < K) ? O[static_cast<size_t>(r) * K + c0 + cc] : R(0);
-                }
-                __syncthreads();
-                const int ce = min(KC, K - c0);
-                for (int cc = 0; cc < ce; ++cc) {
-                    R mv[CPT];
-                    for (int jj = 0; jj < CPT; ++jj) mv[jj] = sM[(c0 + cc) * SMS + tc + 8 * jj];
-#pragma unroll
-                    for (int i = 0; i < 6; ++i) {
-                        const R ov = sOp[cc * SOS + tr + 32 * i];
-#pragma unroll
-                        for (int jj = 0; jj < CPT; ++jj) acc[i][jj] += ov * mv[jj];
-                    }
-                }
-            }
-        }
+        tgt[tid] = t;
     }
     __syncthreads();
+    for (int i = tid; i < 27 * TB; i += 256) {
+        const int d = i / TB, t = i % TB;
+        const int pi = chunk * TB + t;
+        int32_t cb = -1;
+        uint32_t msk = 0;
+        if (tgt[t] >= 0) {
+            const int32_t sp = nbr27[27 * static_cast<int64_t>(parents[pi]) + d];
+            cb = sp >= 0 ? child[sp] : -1;
+            if (cb >= 0)
+                for (int q = 0; q < 8; ++q) msk |= (count[cb + q] != 0 ? 1u : 0u) << q;
+        }
+        s_nc[i] = cb;
+        s_nm[i] = static_cast<uint8_t>(msk);
+    }
+    __syncthreads();
+    if (tid < 189) {
+        const int dn = s_dn[tid], qs = s_qs[tid];
+        int live = 0;
+        for (int t = 0; t < TB; ++t) live |= (s_nm[dn * TB + t] >> qs) & 1;
+        s_live[tid] = static_cast<int16_t>(live);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int nl = 0;
+        for (int st = 0; st < 189; ++st)
+            if (s_live[st]) s_live[nl++] = static_cast<int16_t>(st);
+        s_nlive = nl;
+    }
+    __syncthreads();
+    const int nlive = s_nlive;
+    R acc[RPT][CPT];
+#pragma unroll
+    for (int i = 0; i < RPT; ++i)
+#pragma unroll
+        for (int jj = 0; jj < CPT; ++jj) acc[i][jj] = 0;
+    const int nkc = (K + KC - 1) / KC;
+    const size_t KK = static_cast<size_t>(K) * K;
+    R pf[NPF];
+    // operator chunk (offset step li, columns [c0, c0 + KC)) into registers:
+    // word u of thread tid is row r = i / KC, column c0 + i % KC, i = tid + 256 u
+    auto load_op = [&](int li, int c0) {
+        const R *O = op + (16 + s_o[s_live[li]]) * KK;
+#pragma unroll
+        for (int u = 0; u < NPF; ++u) {
+            const int i = tid + 256 * u, r = i / KC, c = c0 + i % KC;
+            pf[u] = (r < K && c < K) ? O[static_cast<size_t>(r) * K + c] : R(0);
+        }
+    };
+    auto store_op = [&](int b) {
+        R *S = sOp + b * KC * SOS;
+#pragma unroll
+        for (int u = 0; u < NPF; ++u) {
+            const int i = tid + 256 * u;
+            S[(i % KC) * SOS + i / KC] = pf[u];
+        }
+    };
+    auto fma_col = [&](const R *S, int col) {
+        R mv[CPT];
+#pragma unroll
+        for (int v = 0; v < CPT / VW; ++v) {
+            typedef R vec __attribute__((ext_vector_type(VW)));
+            const vec x = *reinterpret_cast<const vec *>(sM + col * SMS + tc * CPT + v * VW);
+#pragma unroll
+            for (int e = 0; e < VW; ++e) mv[v * VW + e] = x[e];
+        }
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) {
+            const R ov = S[tr + 32 * i];
+#pragma unroll
+            for (int jj = 0; jj < CPT; ++jj) acc[i][jj] += ov * mv[jj];
+        }
+    };
+    if (nlive > 0) load_op(0, 0);
+    const int wv = tid >> 6, lane = tid & 63;
+    for (int li = 0; li < nlive; ++li) {
+        const int st = s_live[li], dn = s_dn[st], qs = s_qs[st];
+        // gather: wave wv takes targets wv, wv + 4, ..., lanes the columns
+        for (int t = wv; t < TB; t += 4) {
+            const int32_t cb = s_nc[dn * TB + t];
+            const bool has = (s_nm[dn * TB + t] >> qs) & 1;
+            const R *src = Mx + static_cast<size_t>(has ? cb + qs : 0) * ldk;
+            for (int col = lane; col < K; col += 64) sM[col * SMS + t] = has ? sg[col] * src[col] : R(0);
+        }
+        store_op(0);
+        if (nkc > 1) load_op(li, KC);
+        else if (li + 1 < nlive) load_op(li + 1, 0);
+        __syncthreads();  // multipoles and chunk 0 are in LDS
+        for (int kc = 0; kc < nkc; ++kc) {
+            const int c0 = kc * KC, b = kc & 1;
+            const R *S = sOp + b * KC * SOS;
+            if (c0 + KC <= K) {
+#pragma unroll
+                for (int cc = 0; cc < KC; ++cc) fma_col(S + cc * SOS, c0 + cc);
+            } else {
+                for (int cc = 0; cc < K - c0; ++cc) fma_col(S + cc * SOS, c0 + cc);
+            }
+            if (kc + 1 < nkc) {
+                store_op(b ^ 1);
+                if (kc + 2 < nkc) load_op(li, c0 + 2 * KC);
+                else if (li + 1 < nlive) load_op(li + 1, 0);
+            }
+            __syncthreads();  // chunk kc is read; chunk kc + 1 is in LDS
+        }
+    }
+#pragma unroll
     for (int jj = 0; jj < CPT; ++jj) {
-        const int32_t t = tgt[tc + 8 * jj];
+        const int32_t t = tgt[tc * CPT + jj];
         if (t < 0) continue;
-        for (int i = 0; i < 6; ++i) {
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) {
             const int r = tr + 32 * i;
-            if (r < K) Lx[static_cast<size_t>(t) * ldk + r] = acc[i][jj];
+            if (r < K) Lx[static_cast<size_t>(t) * ldk + r] = sg[r] * acc[i][jj];
         }
     }
 }
@@ -756,14 +830,40 @@ void upward_t(Ctx &c) {
     VG_LAUNCH_CHECK();
 }
 
+template <class R, int TB, int RPT>
+void m2l_plain_launch(Ctx &c, const int32_t *parents, int np, unsigned grid) {
+    constexpr int KC = sizeof(R) == 4 ? 16 : 8;
+    const size_t lds = (static_cast<size_t>(c.K) * (TB + 16 / sizeof(R)) + 2 * KC * (32 * RPT + 1)) * sizeof(R);
+    // dynamic LDS above 64 KiB is allowed per kernel, here for the largest K of this RPT
+    static const bool attr = [] {
+        const size_t cap = (static_cast<size_t>(32 * RPT) * (TB + 16 / sizeof(R)) + 2 * KC * (32 * RPT + 1)) * sizeof(R);
+        VG_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_m2l_plain<R, TB, RPT>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   static_cast<int>(cap)));
+        return true;
+    }();
+    (void)attr;
+    const TreeDev &t = c.t;
+    k_m2l_plain<R, TB, RPT><<<grid, 256, lds, c.stream>>>(parents, np, t.child, t.count, t.nbr27, Bufs<R>::M(c), Bufs<R>::L(c), Bufs<R>::op(c), c.K, c.ldk);
+    VG_LAUNCH_CHECK();
+}
+
 template <class R>
 void m2l_plain_list_t(Ctx &c, const int32_t *parents, int np) {
     const TreeDev &t = c.t;
     if (np <= 0) return;
-    constexpr int TB = sizeof(R) == 4 ? 32 : 16;
-    dim3 grid(ceil_div(np, TB), 8);
-    k_m2l_plain<R, TB><<<grid, 256, 0, c.stream>>>(parents, np, t.child, t.count, t.nbr27, Bufs<R>::M(c), Bufs<R>::L(c), Bufs<R>::op(c), c.K, c.ldk);
-    VG_LAUNCH_CHECK();
+    constexpr int TB = sizeof(R) == 4 ? 64 : 32;
+    const int rpt = ceil_div(c.K, 32);
+    const unsigned grid = 8 * ceil_div(np, TB);  // one block per (parent chunk, child octant)
+    switch (rpt) {
+    case 1: m2l_plain_launch<R, TB, 1>(c, parents, np, grid); break;
+    case 2: m2l_plain_launch<R, TB, 2>(c, parents, np, grid); break;
+    case 3: m2l_plain_launch<R, TB, 3>(c, parents, np, grid); break;
+    case 4: m2l_plain_launch<R, TB, 4>(c, parents, np, grid); break;
+    case 5: m2l_plain_launch<R, TB, 5>(c, parents, np, grid); break;
+    case 6: m2l_plain_launch<R, TB, 6>(c, parents, np, grid); break;
+    default: throw std::runtime_error("plain M2L: expansion order out of range");
+    }
+    (void)t;
 }
 
 template <class R>

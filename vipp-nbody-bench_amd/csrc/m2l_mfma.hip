@@ -118,33 +118,8 @@ __global__ __launch_bounds__(64 * NG * mt_of((P + 1) * (P + 1))) void k_m2l_spli
     // (-1)^m on cosine, -(-1)^m on sine coefficients; y -> -y: -1 on sine;
     // z -> -z: (-1)^(n+m)).  So L = D_qt sum_e T_e (D_qt M): the signs go on the
     // gathered multipoles and on the result, and all blocks share 189 operators.
-    if (tid == 0) {
-        int st = 0;
-        for (int d = 0; d < 27; ++d) {
-            const int dx = d % 3 - 1, dy = (d / 3) % 3 - 1, dz = d / 9 - 1;
-            for (int q = 0; q < 8; ++q) {
-                const int ex = 2 * dx - (q & 1), ey = 2 * dy - ((q >> 1) & 1), ez = 2 * dz - ((q >> 2) & 1);
-                if (ex >= -1 && ex <= 1 && ey >= -1 && ey <= 1 && ez >= -1 && ez <= 1) continue;  // adjacent: near field
-                const int mx = (qt & 1) ? -dx : dx, my = (qt & 2) ? -dy : dy, mz = (qt & 4) ? -dz : dz;
-                s_o[st] = static_cast<int16_t>((ex + 3) + 7 * (ey + 3) + 49 * (ez + 3));
-                s_dn[st] = static_cast<int16_t>((mx + 1) + 3 * (my + 1) + 9 * (mz + 1));
-                s_qs[st] = static_cast<int16_t>(q ^ qt);
-                ++st;
-            }
-        }
-    }
-    for (int k = tid; k < 16 * NKS; k += NTH) {
-        int nn = 0;  // k = n^2 + r: degree n, r = 0 (m = 0), 2m - 1 (cosine) or 2m (sine)
-        while ((nn + 1) * (nn + 1) <= k) ++nn;
-        const int r = k - nn * nn;
-        const int m = (r + 1) >> 1;
-        const bool im = r > 0 && (r & 1) == 0;
-        int sg = 1;
-        if (qt & 1) sg *= ((m & 1) ? -1 : 1) * (im ? -1 : 1);
-        if ((qt & 2) && im) sg = -sg;
-        if ((qt & 4) && ((nn + m) & 1)) sg = -sg;
-        s_sgn[k] = static_cast<float>(sg);
-    }
+    if (tid == 0) m2l_walk(qt, s_o, s_dn, s_qs);
+    for (int k = tid; k < 16 * NKS; k += NTH) s_sgn[k] = static_cast<float>(m2l_mirror_sign(qt, k));
     __syncthreads();
     for (int k8 = tid; k8 < 2 * NKS; k8 += NTH) {
         uint32_t w[4];
