@@ -333,6 +333,24 @@ def test_split_m2l_locals_match_plain(vg, orc):
     assert (np.abs(a - b) / scale).max() < 1e-4
 
 
+@pytest.mark.parametrize("p", [3, 5, 9, 12])
+def test_plain_m2l_locals_match_f64(vg, orc, p):
+    """The FP32 VALU M2L (1, 2, 4 and 6 row groups of 32; the even ones on two
+    thread groups) gives the FP64 path's local expansions to FP32 accuracy,
+    cell by cell, on a clustered input whose blocks hold empty octants."""
+    x, _, _ = orc.problem_initialize(40000, 9, 1e-3)
+    x[:20000] *= 0.05  # a dense core inside a sparse halo
+    Ls = {}
+    for mode in ("f64", "f32"):
+        with vg.Gravity(len(x), 32, p, mode, eps=1e-3) as gr:
+            gr.upload(x)
+            gr.accelerations(copy=False)
+            Ls[mode] = gr.expansions()[1]
+    a, b = Ls["f64"], Ls["f32"]
+    scale = np.abs(a).max(axis=(1, 2), keepdims=True) + 1e-300
+    assert (np.abs(a - b) / scale).max() < 2e-5
+
+
 def test_energy_f64(vg, orc):
     x, v, _ = orc.problem_initialize(5000, 6, 1e-3)
     _, pot = orc.direct(x, eps=1e-3, potential=True)

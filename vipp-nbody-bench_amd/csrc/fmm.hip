@@ -420,13 +420,14 @@ __global__ __launch_bounds__(256) void k_l2l(const int32_t *__restrict__ ids, in
 // octant blocks of a parent chunk run side by side on one XCD and share its
 // L2.  The prologue tabulates, per parent neighbour and target, the first
 // child and the mask of non-empty children, and from them the offsets that
-// hold any source; only those are walked.  A thread owns RPT rows
-// (tr + 32 i) and CPT consecutive targets; the operator streams through LDS
-// in chunks of KC columns, double-buffered through registers (one barrier per
-// chunk), and the next offset's first chunk is fetched during the last.
+// hold any source; only those are walked.  A block has RG groups of 256
+// threads; a thread owns the rows tr + 32 (rg + RG i), i < RPT / RG, and CPT
+// consecutive targets; the operator streams through LDS in chunks of KC
+// columns, double-buffered through registers (one barrier per chunk), and the
+// next offset's first chunk is fetched during the last.
 // ---------------------------------------------------------------------------
-template <class R, int TB, int RPT>
-__global__ __launch_bounds__(256) void k_m2l_plain(const int32_t *__restrict__ parents, int nparents, const int32_t *__restrict__ child,
+template <class R, int TB, int RPT, int RG>
+__global__ __launch_bounds__(256 * RG) void k_m2l_plain(const int32_t *__restrict__ parents, int nparents, const int32_t *__restrict__ child,
                                                    const uint32_t *__restrict__ count, const int32_t *__restrict__ nbr27, const R *__restrict__ Mx,
                                                    R *__restrict__ Lx, const R *__restrict__ op, int K, int ldk) {
     constexpr int KC = sizeof(R) == 4 ? 16 : 8;     // operator columns per chunk
@@ -434,8 +435,10 @@ __global__ __launch_bounds__(256) void k_m2l_plain(const int32_t *__restrict__ p
     constexpr int VW = 16 / sizeof(R);               // elements per 16-byte LDS read
     constexpr int SMS = TB + VW;                     // multipole column stride (16-byte rows)
     constexpr int SOS = 32 * RPT + 1;                // operator column stride (odd: no bank conflicts)
-    constexpr int NPF = KC * 32 * RPT / 256;         // prefetched operator words per thread
-    static_assert(CPT % VW == 0 && (KC * 32 * RPT) % 256 == 0, "tile shape");
+    constexpr int NTH = 256 * RG;                    // threads
+    constexpr int RPL = RPT / RG;                    // rows per thread
+    constexpr int NPF = KC * 32 * RPT / NTH;         // prefetched operator words per thread
+    static_assert(CPT % VW == 0 && (KC * 32 * RPT) % NTH == 0 && RPT % RG == 0, "tile shape");
     extern __shared__ __align__(16) unsigned char m2l_smem[];
     R *sM = reinterpret_cast<R *>(m2l_smem);         // K x SMS: column k of the gathered, signed multipoles
     R *sOp = sM + K * SMS;                           // 2 x KC x SOS: operator columns, transposed
@@ -449,9 +452,9 @@ __global__ __launch_bounds__(256) void k_m2l_plain(const int32_t *__restrict__ p
     const int w = (blockIdx.x & 7) * nchunks + (blockIdx.x >> 3);  // XCD x takes work items [x W / 8, (x + 1) W / 8)
     if (w >= 8 * nchunks) return;
     const int chunk = w >> 3, qt = w & 7;
-    const int tid = threadIdx.x, tr = tid & 31, tc = tid >> 5;
+    const int tid = threadIdx.x, tr = tid & 31, tc = (tid >> 5) & 7, rg = tid >> 8;
     if (tid == 0) m2l_walk(qt, s_o, s_dn, s_qs);
-    for (int k = tid; k < 32 * RPT; k += 256) sg[k] = static_cast<R>(k < K ? m2l_mirror_sign(qt, k) : 0);
+    for (int k = tid; k < 32 * RPT; k += NTH) sg[k] = static_cast<R>(k < K ? m2l_mirror_sign(qt, k) : 0);
     if (tid < TB) {
         const int pi = chunk * TB + tid;
         const int32_t P = pi < nparents ? parents[pi] : -1;
@@ -460,7 +463,7 @@ __global__ __launch_bounds__(256) void k_m2l_plain(const int32_t *__restrict__ p
         tgt[tid] = t;
     }
     __syncthreads();
-    for (int i = tid; i < 27 * TB; i += 256) {
+    for (int i = tid; i < 27 * TB; i += NTH) {
         const int d = i / TB, t = i % TB;
         const int pi = chunk * TB + t;
         int32_t cb = -1;
@@ -490,21 +493,21 @@ __global__ __launch_bounds__(256) void k_m2l_plain(const int32_t *__restrict__ p
     }
     __syncthreads();
     const int nlive = s_nlive;
-    R acc[RPT][CPT];
+    R acc[RPL][CPT];
 #pragma unroll
-    for (int i = 0; i < RPT; ++i)
+    for (int i = 0; i < RPL; ++i)
 #pragma unroll
         for (int jj = 0; jj < CPT; ++jj) acc[i][jj] = 0;
     const int nkc = (K + KC - 1) / KC;
     const size_t KK = static_cast<size_t>(K) * K;
     R pf[NPF];
     // operator chunk (offset step li, columns [c0, c0 + KC)) into registers:
-    // word u of thread tid is row r = i / KC, column c0 + i % KC, i = tid + 256 u
+    // word u of thread tid is row r = i / KC, column c0 + i % KC, i = tid + NTH u
     auto load_op = [&](int li, int c0) {
         const R *O = op + (16 + s_o[s_live[li]]) * KK;
 #pragma unroll
         for (int u = 0; u < NPF; ++u) {
-            const int i = tid + 256 * u, r = i / KC, c = c0 + i % KC;
+            const int i = tid + NTH * u, r = i / KC, c = c0 + i % KC;
             pf[u] = (r < K && c < K) ? O[static_cast<size_t>(r) * K + c] : R(0);
         }
     };
@@ -512,7 +515,7 @@ __global__ __launch_bounds__(256) void k_m2l_plain(const int32_t *__restrict__ p
         R *S = sOp + b * KC * SOS;
 #pragma unroll
         for (int u = 0; u < NPF; ++u) {
-            const int i = tid + 256 * u;
+            const int i = tid + NTH * u;
             S[(i % KC) * SOS + i / KC] = pf[u];
         }
     };
@@ -526,8 +529,8 @@ __global__ __launch_bounds__(256) void k_m2l_plain(const int32_t *__restrict__ p
             for (int e = 0; e < VW; ++e) mv[v * VW + e] = x[e];
         }
 #pragma unroll
-        for (int i = 0; i < RPT; ++i) {
-            const R ov = S[tr + 32 * i];
+        for (int i = 0; i < RPL; ++i) {
+            const R ov = S[tr + 32 * (rg + RG * i)];
 #pragma unroll
             for (int jj = 0; jj < CPT; ++jj) acc[i][jj] += ov * mv[jj];
         }
@@ -536,8 +539,8 @@ __global__ __launch_bounds__(256) void k_m2l_plain(const int32_t *__restrict__ p
     const int wv = tid >> 6, lane = tid & 63;
     for (int li = 0; li < nlive; ++li) {
         const int st = s_live[li], dn = s_dn[st], qs = s_qs[st];
-        // gather: wave wv takes targets wv, wv + 4, ..., lanes the columns
-        for (int t = wv; t < TB; t += 4) {
+        // gather: wave wv takes targets wv, wv + 4 RG, ..., lanes the columns
+        for (int t = wv; t < TB; t += 4 * RG) {
             const int32_t cb = s_nc[dn * TB + t];
             const bool has = (s_nm[dn * TB + t] >> qs) & 1;
             const R *src = Mx + static_cast<size_t>(has ? cb + qs : 0) * ldk;
@@ -569,8 +572,8 @@ __global__ __launch_bounds__(256) void k_m2l_plain(const int32_t *__restrict__ p
         const int32_t t = tgt[tc * CPT + jj];
         if (t < 0) continue;
 #pragma unroll
-        for (int i = 0; i < RPT; ++i) {
-            const int r = tr + 32 * i;
+        for (int i = 0; i < RPL; ++i) {
+            const int r = tr + 32 * (rg + RG * i);
             if (r < K) Lx[static_cast<size_t>(t) * ldk + r] = sg[r] * acc[i][jj];
         }
     }
@@ -830,20 +833,20 @@ void upward_t(Ctx &c) {
     VG_LAUNCH_CHECK();
 }
 
-template <class R, int TB, int RPT>
+template <class R, int TB, int RPT, int RG = 1>
 void m2l_plain_launch(Ctx &c, const int32_t *parents, int np, unsigned grid) {
     constexpr int KC = sizeof(R) == 4 ? 16 : 8;
     const size_t lds = (static_cast<size_t>(c.K) * (TB + 16 / sizeof(R)) + 2 * KC * (32 * RPT + 1)) * sizeof(R);
     // dynamic LDS above 64 KiB is allowed per kernel, here for the largest K of this RPT
     static const bool attr = [] {
         const size_t cap = (static_cast<size_t>(32 * RPT) * (TB + 16 / sizeof(R)) + 2 * KC * (32 * RPT + 1)) * sizeof(R);
-        VG_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_m2l_plain<R, TB, RPT>), hipFuncAttributeMaxDynamicSharedMemorySize,
+        VG_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_m2l_plain<R, TB, RPT, RG>), hipFuncAttributeMaxDynamicSharedMemorySize,
                                    static_cast<int>(cap)));
         return true;
     }();
     (void)attr;
     const TreeDev &t = c.t;
-    k_m2l_plain<R, TB, RPT><<<grid, 256, lds, c.stream>>>(parents, np, t.child, t.count, t.nbr27, Bufs<R>::M(c), Bufs<R>::L(c), Bufs<R>::op(c), c.K, c.ldk);
+    k_m2l_plain<R, TB, RPT, RG><<<grid, 256 * RG, lds, c.stream>>>(parents, np, t.child, t.count, t.nbr27, Bufs<R>::M(c), Bufs<R>::L(c), Bufs<R>::op(c), c.K, c.ldk);
     VG_LAUNCH_CHECK();
 }
 
@@ -852,15 +855,18 @@ void m2l_plain_list_t(Ctx &c, const int32_t *parents, int np) {
     const TreeDev &t = c.t;
     if (np <= 0) return;
     constexpr int TB = sizeof(R) == 4 ? 64 : 32;
+    // FP32 with an even number of 32-row groups: two thread groups split the
+    // rows (512 threads, twice the wavefronts per CU within the same LDS)
+    constexpr int RG2 = sizeof(R) == 4 ? 2 : 1;
     const int rpt = ceil_div(c.K, 32);
     const unsigned grid = 8 * ceil_div(np, TB);  // one block per (parent chunk, child octant)
     switch (rpt) {
     case 1: m2l_plain_launch<R, TB, 1>(c, parents, np, grid); break;
-    case 2: m2l_plain_launch<R, TB, 2>(c, parents, np, grid); break;
+    case 2: m2l_plain_launch<R, TB, 2, RG2>(c, parents, np, grid); break;
     case 3: m2l_plain_launch<R, TB, 3>(c, parents, np, grid); break;
-    case 4: m2l_plain_launch<R, TB, 4>(c, parents, np, grid); break;
+    case 4: m2l_plain_launch<R, TB, 4, RG2>(c, parents, np, grid); break;
     case 5: m2l_plain_launch<R, TB, 5>(c, parents, np, grid); break;
-    case 6: m2l_plain_launch<R, TB, 6>(c, parents, np, grid); break;
+    case 6: m2l_plain_launch<R, TB, 6, RG2>(c, parents, np, grid); break;
     default: throw std::runtime_error("plain M2L: expansion order out of range");
     }
     (void)t;
