@@ -422,43 +422,64 @@ __global__ __launch_bounds__(1024) void k_flag_write_lv(const uint8_t *__restric
     if (f && pos < cap_out) out[pos] = static_cast<int32_t>(a.base + i);
 }
 
-// The three kernels above in one block, for levels with few flags: ordered
-// compaction in chunks of 1024, then the level table updated as
-// k_count_scan_lv does.
-__global__ __launch_bounds__(1024) void k_compact1_lv(const uint8_t *__restrict__ flag, uint32_t *lv, int d, int32_t *__restrict__ out,
-                                                      int64_t cap_out, bool last) {
-    __shared__ uint32_t wsum[16];
-    __shared__ uint32_t carry;
-    const LvCmp a = lv_cmp(lv, d);  // every thread reads lv before the barrier; thread 0 writes it only at the end
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    const unsigned lane = threadIdx.x & 63;
-    for (int64_t off = 0; off < a.nflag; off += 1024) {
-        const int64_t i = off + threadIdx.x;
-        const bool f = i < a.nflag && flag[i];
-        const uint64_t bal = __ballot(f);
-        if (lane == 0) wsum[threadIdx.x >> 6] = __popcll(bal);
-        __syncthreads();
-        uint32_t wb = 0;
-        for (unsigned w = 0; w < (threadIdx.x >> 6); ++w) wb += wsum[w];
-        const uint32_t c0 = carry;
-        const int64_t pos = a.out0 + c0 + wb + __popcll(bal & ((1ull << lane) - 1));
-        if (f && pos < cap_out) out[pos] = static_cast<int32_t>(a.base + i);
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t tot = 0;
-            for (int w = 0; w < 16; ++w) tot += wsum[w];
-            carry = c0 + tot;
-        }
-        __syncthreads();
+// Set flag bytes (each 0 or 1) among the first rem (>= 1) bytes of w.
+__device__ inline uint32_t flag_count16(const uint4 w, int64_t rem) {
+    const uint32_t v[4] = {w.x, w.y, w.z, w.w};
+    uint32_t c = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int64_t r = rem - 4 * j;
+        const uint32_t m = r >= 4 ? 0xffffffffu : (r <= 0 ? 0u : (1u << (8 * r)) - 1u);
+        c += __builtin_popcount(v[j] & m);
     }
-    if (threadIdx.x == 0) {
-        const uint32_t tot = carry;
-        const uint32_t nint = lv[LV_I + d + 1] - lv[LV_I + d];
-        lv[LV_I + d + 2] = lv[LV_I + d + 1] + tot;
-        lv[d + 2] = lv[d + 1] + 8u * nint;
-        if (lv[LV_I + d + 2] > cap_out) lv[LV_OVF] |= 1u;
-        if (last && tot > 0) lv[LV_OVF] |= 2u;
+    return c;
+}
+
+// k_flag_counts_lv, k_count_scan_lv and k_flag_write_lv in one block, for up
+// to WIDE_FLAGS flags: thread t counts the flags of the contiguous run
+// [t per, (t + 1) per) 16 bytes at a time, the block scans the counts once, and
+// each thread writes its run's entries in order; then the level table is
+// updated as k_count_scan_lv does.
+constexpr int64_t WIDE_FLAGS = 1 << 21;
+__global__ __launch_bounds__(1024) void k_compact_wide_lv(const uint8_t *__restrict__ flag, uint32_t *lv, int d, int32_t *__restrict__ out,
+                                                          int64_t cap_out, bool last) {
+    __shared__ uint32_t wsum[16];
+    const LvCmp a = lv_cmp(lv, d);  // every thread reads lv before the barrier; the last thread writes it after
+    const int64_t per = ((a.nflag + 1023) / 1024 + 15) / 16 * 16;
+    const int64_t b0 = threadIdx.x * per, b1 = min(a.nflag, b0 + per);
+    uint32_t cnt = 0;
+    for (int64_t i = b0; i < b1; i += 16) cnt += flag_count16(*reinterpret_cast<const uint4 *>(flag + i), b1 - i);
+    const unsigned lane = threadIdx.x & 63;
+    uint32_t x = cnt;
+    for (int s = 1; s < 64; s <<= 1) {
+        const uint32_t o = __shfl_up(x, s, 64);
+        if (lane >= static_cast<unsigned>(s)) x += o;
+    }
+    if (lane == 63) wsum[threadIdx.x >> 6] = x;
+    __syncthreads();
+    uint32_t wb = 0;
+    for (unsigned w = 0; w < (threadIdx.x >> 6); ++w) wb += wsum[w];
+    int64_t pos = a.out0 + wb + x - cnt;
+    for (int64_t i = b0; i < b1; i += 16) {
+        const uint4 w = *reinterpret_cast<const uint4 *>(flag + i);
+        const uint32_t v[4] = {w.x, w.y, w.z, w.w};
+        for (int k = 0; k < 16 && i + k < b1; ++k)
+            if ((v[k >> 2] >> (8 * (k & 3))) & 0xffu) {
+                if (pos < cap_out) out[pos] = static_cast<int32_t>(a.base + i + k);
+                ++pos;
+            }
+    }
+    if (threadIdx.x == 1023) {
+        const uint32_t tot = wb + x;
+        if (d >= 0) {
+            const uint32_t nint = lv[LV_I + d + 1] - lv[LV_I + d];
+            lv[LV_I + d + 2] = lv[LV_I + d + 1] + tot;
+            lv[d + 2] = lv[d + 1] + 8u * nint;
+            if (lv[LV_I + d + 2] > cap_out) lv[LV_OVF] |= 1u;
+            if (last && tot > 0) lv[LV_OVF] |= 2u;  // cells of the last sorted level split further
+        } else {
+            lv[LV_NLEAF] = tot;
+        }
     }
 }
 
@@ -771,8 +792,8 @@ static int tree_build_lv(Ctx &c, int SH, uint64_t *&k0, uint32_t *&v0, uint64_t 
         const unsigned gc = static_cast<unsigned>(ceil_div(nflag[d], 1024));
         k_expand_lv<<<ceil_div(nflag[d], 256), 256, 0, st>>>(d, t.lv, t.int_ids, cap, k0, c.thr, t.start, t.count, t.child, t.parent, t.octant,
                                                               t.level, t.bmin, t.bmax, t.grid, t.anc, t.flags);
-        if (nflag[d] <= 8192) {  // one launch instead of three
-            k_compact1_lv<<<1, 1024, 0, st>>>(t.flags, t.lv, d, t.int_ids, cap, d == SH - 1);
+        if (nflag[d] <= WIDE_FLAGS) {  // one launch instead of three
+            k_compact_wide_lv<<<1, 1024, 0, st>>>(t.flags, t.lv, d, t.int_ids, cap, d == SH - 1);
         } else {
             k_flag_counts_lv<<<gc, 1024, 0, st>>>(t.flags, t.lv, d, c.cmp_cnt);
             k_count_scan_lv<<<1, 1024, 0, st>>>(c.cmp_cnt, static_cast<int>(gc), t.lv, d, cap, d == SH - 1);
@@ -783,9 +804,13 @@ static int tree_build_lv(Ctx &c, int SH, uint64_t *&k0, uint32_t *&v0, uint64_t 
     // non-empty leaves among the lv[SH + 1] cells
     const unsigned gl = static_cast<unsigned>(ceil_div(cap, 1024));
     k_leaf_flags_lv<<<ceil_div(cap, 256), 256, 0, st>>>(t.child, t.count, t.lv, SH + 1, cap, t.flags);
-    k_flag_counts_lv<<<gl, 1024, 0, st>>>(t.flags, t.lv, -(SH + 2), c.cmp_cnt);
-    k_count_scan_lv<<<1, 1024, 0, st>>>(c.cmp_cnt, static_cast<int>(gl), t.lv, -(SH + 2), cap, false);
-    k_flag_write_lv<<<gl, 1024, 0, st>>>(t.flags, t.lv, -(SH + 2), c.cmp_cnt, t.leaf_ids, cap);
+    if (cap <= WIDE_FLAGS) {
+        k_compact_wide_lv<<<1, 1024, 0, st>>>(t.flags, t.lv, -(SH + 2), t.leaf_ids, cap, false);
+    } else {
+        k_flag_counts_lv<<<gl, 1024, 0, st>>>(t.flags, t.lv, -(SH + 2), c.cmp_cnt);
+        k_count_scan_lv<<<1, 1024, 0, st>>>(c.cmp_cnt, static_cast<int>(gl), t.lv, -(SH + 2), cap, false);
+        k_flag_write_lv<<<gl, 1024, 0, st>>>(t.flags, t.lv, -(SH + 2), c.cmp_cnt, t.leaf_ids, cap);
+    }
     c.perm.ensure(std::max<int64_t>(1, n));
     // a wavefront per leaf of the previous tree and a quarter more (the leaf
     // count changes little; the blocks loop over any excess)
