@@ -153,8 +153,11 @@ __global__ __launch_bounds__(256) void k_expand(int d, const int32_t *__restrict
     const int32_t P = ids[k];
     const uint32_t s = start[P], e = s + count[P];
     const int shift = 3 * (MAXD - 1 - d);
+    // child q ends where child q + 1 starts: the 8 children of a parent are 8
+    // consecutive lanes (8-aligned), so one search per lane and a shuffle
     const uint32_t lo = digit_lower_bound(key, s, e, shift, q);
-    const uint32_t hi = q == 7 ? e : digit_lower_bound(key, lo, e, shift, q + 1);
+    const uint32_t nx = __shfl_down(lo, 1, 8);
+    const uint32_t hi = q == 7 ? e : nx;
     const int64_t c = child_off + 8ll * k + q;
     start[c] = lo;
     count[c] = hi - lo;
@@ -317,8 +320,11 @@ __global__ __launch_bounds__(256) void k_expand_lv(int d, uint32_t *lv, const in
     const int32_t P = int_ids[i0 + k];
     const uint32_t s = start[P], e = s + count[P];
     const int shift = 3 * (MAXD - 1 - d);
+    // child q ends where child q + 1 starts: the 8 children of a parent are 8
+    // consecutive lanes (8-aligned), so one search per lane and a shuffle
     const uint32_t lo = digit_lower_bound(key, s, e, shift, q);
-    const uint32_t hi = q == 7 ? e : digit_lower_bound(key, lo, e, shift, q + 1);
+    const uint32_t nx = __shfl_down(lo, 1, 8);
+    const uint32_t hi = q == 7 ? e : nx;
     const int64_t c = child_off + 8ll * k + q;
     start[c] = lo;
     count[c] = hi - lo;
@@ -439,7 +445,9 @@ __device__ inline uint32_t flag_count16(const uint4 w, int64_t rem) {
 // to WIDE_FLAGS flags: thread t counts the flags of the contiguous run
 // [t per, (t + 1) per) 16 bytes at a time, the block scans the counts once, and
 // each thread writes its run's entries in order; then the level table is
-// updated as k_count_scan_lv does.
+// updated as k_count_scan_lv does.  The writes of one wavefront's lanes are
+// strided, so this suits the internal cells of a level (a small fraction of
+// the flags), not the leaves.
 constexpr int64_t WIDE_FLAGS = 1 << 21;
 __global__ __launch_bounds__(1024) void k_compact_wide_lv(const uint8_t *__restrict__ flag, uint32_t *lv, int d, int32_t *__restrict__ out,
                                                           int64_t cap_out, bool last) {
@@ -803,14 +811,11 @@ static int tree_build_lv(Ctx &c, int SH, uint64_t *&k0, uint32_t *&v0, uint64_t 
     VG_LAUNCH_CHECK();
     // non-empty leaves among the lv[SH + 1] cells
     const unsigned gl = static_cast<unsigned>(ceil_div(cap, 1024));
+    // (most cells are leaves: the multi-block compaction writes them coalesced)
     k_leaf_flags_lv<<<ceil_div(cap, 256), 256, 0, st>>>(t.child, t.count, t.lv, SH + 1, cap, t.flags);
-    if (cap <= WIDE_FLAGS) {
-        k_compact_wide_lv<<<1, 1024, 0, st>>>(t.flags, t.lv, -(SH + 2), t.leaf_ids, cap, false);
-    } else {
-        k_flag_counts_lv<<<gl, 1024, 0, st>>>(t.flags, t.lv, -(SH + 2), c.cmp_cnt);
-        k_count_scan_lv<<<1, 1024, 0, st>>>(c.cmp_cnt, static_cast<int>(gl), t.lv, -(SH + 2), cap, false);
-        k_flag_write_lv<<<gl, 1024, 0, st>>>(t.flags, t.lv, -(SH + 2), c.cmp_cnt, t.leaf_ids, cap);
-    }
+    k_flag_counts_lv<<<gl, 1024, 0, st>>>(t.flags, t.lv, -(SH + 2), c.cmp_cnt);
+    k_count_scan_lv<<<1, 1024, 0, st>>>(c.cmp_cnt, static_cast<int>(gl), t.lv, -(SH + 2), cap, false);
+    k_flag_write_lv<<<gl, 1024, 0, st>>>(t.flags, t.lv, -(SH + 2), c.cmp_cnt, t.leaf_ids, cap);
     c.perm.ensure(std::max<int64_t>(1, n));
     // a wavefront per leaf of the previous tree and a quarter more (the leaf
     // count changes little; the blocks loop over any excess)
