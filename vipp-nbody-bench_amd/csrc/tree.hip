@@ -500,14 +500,22 @@ __global__ __launch_bounds__(256) void k_leaf_flags_lv(const int32_t *__restrict
 }
 
 // k_leaf_sort with a wavefront per leaf (LS_WAVES leaves per block, LDS of
-// LS_WAVES thr words): the leaves hold about s points, so a block per leaf
-// would leave most of its threads idle. nl leaves, or with lv the
-// lv[LV_NLEAF] leaves of a device-driven build (none when it did not complete).
+// LS_WAVES thr words), fused with k_gather: the leaves hold about s points, so
+// a block per leaf would leave most of its threads idle.  nl leaves, or with
+// lv the lv[LV_NLEAF] leaves of a device-driven build (none when it did not
+// complete).  A lane's first LS_REG points are gathered as soon as their
+// caller indices are read, so the random loads of their coordinates overlap
+// the ranking; the points of larger leaves beyond 64 LS_REG are gathered after
+// it.  Each point is written to its sorted slot with the arithmetic of
+// k_gather.
 constexpr int LS_WAVES = 4;
+constexpr int LS_REG = 3;
 constexpr uint32_t LS_MAXTHR = 4096;  // LDS bound of the wavefront form
 __global__ __launch_bounds__(64 * LS_WAVES) void k_leaf_sort_w(const int32_t *__restrict__ leaves, uint32_t nl, const uint32_t *lv,
                                                                const uint32_t *__restrict__ start, const uint32_t *__restrict__ count, uint32_t thr,
-                                                               const uint32_t *__restrict__ in, uint32_t *__restrict__ out) {
+                                                               const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
+                                                               const double *__restrict__ x, const double *__restrict__ m, double mass,
+                                                               const double *__restrict__ box, double *__restrict__ xs64, float *__restrict__ xs32) {
     extern __shared__ uint32_t sv[];
     if (lv) {
         if (lv[LV_OVF]) return;
@@ -515,19 +523,58 @@ __global__ __launch_bounds__(64 * LS_WAVES) void k_leaf_sort_w(const int32_t *__
     }
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     uint32_t *sw = sv + w * thr;
+    const double cx = (box[0] + box[3]) / 2, cy = (box[1] + box[4]) / 2, cz = (box[2] + box[5]) / 2;
+    auto put = [&](uint32_t k, uint32_t i, double px, double py, double pz, double mi) {
+        out[k] = i;
+        xs64[4ll * k] = px;
+        xs64[4ll * k + 1] = py;
+        xs64[4ll * k + 2] = pz;
+        xs64[4ll * k + 3] = mi;
+        xs32[4ll * k] = static_cast<float>(px - cx);
+        xs32[4ll * k + 1] = static_cast<float>(py - cy);
+        xs32[4ll * k + 2] = static_cast<float>(pz - cz);
+        xs32[4ll * k + 3] = static_cast<float>(mi);
+    };
     for (uint32_t j = blockIdx.x * LS_WAVES + w; j < nl; j += gridDim.x * LS_WAVES) {
         const int32_t c = leaves[j];
         const uint32_t s = start[c], n = count[c];
         if (n > thr) continue;  // not a leaf of a complete build (same n for the whole wavefront)
-        for (uint32_t i = lane; i < n; i += 64) sw[i] = in[s + i];
+        uint32_t vi[LS_REG];
+        double px[LS_REG], py[LS_REG], pz[LS_REG], pm[LS_REG];
+#pragma unroll
+        for (int u = 0; u < LS_REG; ++u) {
+            const uint32_t i = lane + 64u * u;
+            vi[u] = i < n ? in[s + i] : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < LS_REG; ++u) {
+            const uint32_t i = lane + 64u * u;
+            if (i < n) {
+                px[u] = x[3ll * vi[u]];
+                py[u] = x[3ll * vi[u] + 1];
+                pz[u] = x[3ll * vi[u] + 2];
+                pm[u] = m ? m[vi[u]] : mass;
+                sw[i] = vi[u];
+            }
+        }
+        for (uint32_t i = lane + 64u * LS_REG; i < n; i += 64) sw[i] = in[s + i];
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        for (uint32_t i = lane; i < n; i += 64) {
+#pragma unroll
+        for (int u = 0; u < LS_REG; ++u) {
+            const uint32_t i = lane + 64u * u;
+            if (i < n) {
+                uint32_t r = 0;
+                for (uint32_t k = 0; k < n; ++k) r += sw[k] < vi[u];
+                put(s + r, vi[u], px[u], py[u], pz[u], pm[u]);
+            }
+        }
+        for (uint32_t i = lane + 64u * LS_REG; i < n; i += 64) {
             const uint32_t v = sw[i];
             uint32_t r = 0;
             for (uint32_t k = 0; k < n; ++k) r += sw[k] < v;
-            out[s + r] = v;
+            put(s + r, v, x[3ll * v], x[3ll * v + 1], x[3ll * v + 2], m ? m[v] : mass);
         }
         __builtin_amdgcn_wave_barrier();  // every lane has read sw before the next leaf overwrites it
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -820,11 +867,10 @@ static int tree_build_lv(Ctx &c, int SH, uint64_t *&k0, uint32_t *&v0, uint64_t 
     // a wavefront per leaf of the previous tree and a quarter more (the leaf
     // count changes little; the blocks loop over any excess)
     const int64_t gls = ceil_div(std::min<int64_t>(cap, std::max<int64_t>(1024, t.nleaves + t.nleaves / 4)), LS_WAVES);
-    k_leaf_sort_w<<<static_cast<unsigned>(gls), 64 * LS_WAVES, LS_WAVES * c.thr * sizeof(uint32_t), st>>>(t.leaf_ids, 0, t.lv, t.start, t.count,
-                                                                                                          c.thr, v0, c.perm);
     c.xs64.ensure(4 * std::max<int64_t>(1, n));
     c.xs32.ensure(4 * std::max<int64_t>(1, n));
-    k_gather<<<ceil_div(n, 256), 256, 0, st>>>(c.perm, n, c.x, c.has_masses ? c.m.p : nullptr, c.mass, c.box, c.xs64, c.xs32, t.lv);
+    k_leaf_sort_w<<<static_cast<unsigned>(gls), 64 * LS_WAVES, LS_WAVES * c.thr * sizeof(uint32_t), st>>>(
+        t.leaf_ids, 0, t.lv, t.start, t.count, c.thr, v0, c.perm, c.x, c.has_masses ? c.m.p : nullptr, c.mass, c.box, c.xs64, c.xs32);
     VG_LAUNCH_CHECK();
     double *hbox = static_cast<double *>(c.pinned);                                         // 8 doubles
     uint32_t *hlv = reinterpret_cast<uint32_t *>(static_cast<char *>(c.pinned) + 512);  // LV_WORDS pinned host words
@@ -953,17 +999,15 @@ void tree_build(Ctx &c) {
     VG_HIP(hipStreamSynchronize(st));
     t.nleaves = hbuf[0];
     c.perm.ensure(std::max<int64_t>(1, n));
-    if (t.nleaves) {
-        if (c.thr <= LS_MAXTHR)
-            k_leaf_sort_w<<<static_cast<unsigned>(ceil_div(t.nleaves, LS_WAVES)), 64 * LS_WAVES, LS_WAVES * c.thr * sizeof(uint32_t), st>>>(
-                t.leaf_ids, static_cast<uint32_t>(t.nleaves), nullptr, t.start, t.count, c.thr, v0, c.perm);
-        else
-            k_leaf_sort<<<static_cast<unsigned>(t.nleaves), 256, c.thr * sizeof(uint32_t), st>>>(t.leaf_ids, t.start, t.count, v0, c.perm);
-        VG_LAUNCH_CHECK();
-    }
     c.xs64.ensure(4 * std::max<int64_t>(1, n));
     c.xs32.ensure(4 * std::max<int64_t>(1, n));
-    if (n) {
+    if (t.nleaves && c.thr <= LS_MAXTHR) {  // sort and gather in one pass
+        k_leaf_sort_w<<<static_cast<unsigned>(ceil_div(t.nleaves, LS_WAVES)), 64 * LS_WAVES, LS_WAVES * c.thr * sizeof(uint32_t), st>>>(
+            t.leaf_ids, static_cast<uint32_t>(t.nleaves), nullptr, t.start, t.count, c.thr, v0, c.perm, c.x, c.has_masses ? c.m.p : nullptr,
+            c.mass, c.box, c.xs64, c.xs32);
+        VG_LAUNCH_CHECK();
+    } else if (t.nleaves) {
+        k_leaf_sort<<<static_cast<unsigned>(t.nleaves), 256, c.thr * sizeof(uint32_t), st>>>(t.leaf_ids, t.start, t.count, v0, c.perm);
         k_gather<<<ceil_div(n, 256), 256, 0, st>>>(c.perm, n, c.x, c.has_masses ? c.m.p : nullptr, c.mass, c.box, c.xs64, c.xs32, nullptr);
         VG_LAUNCH_CHECK();
     }
