@@ -142,6 +142,43 @@ __device__ inline uint32_t digit_lower_bound(const uint64_t *__restrict__ key, u
     return lo;
 }
 
+// A child's row from its parent's: everything of the parent is read before
+// any store, so the loads issue together (the same arrays are written for the
+// new cells in the same kernel, so the compiler keeps loads and stores in
+// program order).
+__device__ __forceinline__ void read_parent(int64_t c, int32_t P, int d, const double *bmin, const double *bmax, const int32_t *grid,
+                                            const int32_t *anc, double *pmn, double *pmx, int32_t *pg, int32_t *pa) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        pmn[a] = bmin[3ll * P + a];
+        pmx[a] = bmax[3ll * P + a];
+        pg[a] = grid[3ll * P + a];
+    }
+#pragma unroll
+    for (int l = 0; l <= MAXD; ++l) pa[l] = l <= d ? anc[static_cast<int64_t>(P) * (MAXD + 1) + l] : (l == d + 1 ? static_cast<int32_t>(c) : -1);
+}
+__device__ __forceinline__ void write_child(int64_t c, int32_t P, unsigned q, int d, uint32_t lo, uint32_t hi, uint32_t *start, uint32_t *count,
+                                            int32_t *child, int32_t *parent, uint8_t *octant, int8_t *level, double *bmin, double *bmax,
+                                            int32_t *grid, int32_t *anc, const double *pmn, const double *pmx, const int32_t *pg,
+                                            const int32_t *pa) {
+    start[c] = lo;
+    count[c] = hi - lo;
+    child[c] = -1;
+    parent[c] = P;
+    octant[c] = static_cast<uint8_t>(q);
+    level[c] = static_cast<int8_t>(d + 1);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double mid = (pmx[a] + pmn[a]) / 2;
+        const bool low = (q >> a) & 1u;
+        bmin[3 * c + a] = low ? pmn[a] : mid;
+        bmax[3 * c + a] = low ? mid : pmx[a];
+        grid[3 * c + a] = 2 * pg[a] + (low ? 0 : 1);
+    }
+#pragma unroll
+    for (int l = 0; l <= MAXD; ++l) anc[c * (MAXD + 1) + l] = pa[l];
+}
+
 // Children of the internal cells of level d (octree.hpp:166-191).
 __global__ __launch_bounds__(256) void k_expand(int d, const int32_t *__restrict__ ids, int nint, int64_t child_off, const uint64_t *__restrict__ key,
                                                 uint32_t thr, uint32_t *start, uint32_t *count, int32_t *child, int32_t *parent, uint8_t *octant,
@@ -152,28 +189,17 @@ __global__ __launch_bounds__(256) void k_expand(int d, const int32_t *__restrict
     const unsigned q = static_cast<unsigned>(t & 7);
     const int32_t P = ids[k];
     const uint32_t s = start[P], e = s + count[P];
+    const int64_t c = child_off + 8ll * k + q;
+    double pmn[3], pmx[3];
+    int32_t pg[3], pa[MAXD + 1];
+    read_parent(c, P, d, bmin, bmax, grid, anc, pmn, pmx, pg, pa);
     const int shift = 3 * (MAXD - 1 - d);
     // child q ends where child q + 1 starts: the 8 children of a parent are 8
     // consecutive lanes (8-aligned), so one search per lane and a shuffle
     const uint32_t lo = digit_lower_bound(key, s, e, shift, q);
     const uint32_t nx = __shfl_down(lo, 1, 8);
     const uint32_t hi = q == 7 ? e : nx;
-    const int64_t c = child_off + 8ll * k + q;
-    start[c] = lo;
-    count[c] = hi - lo;
-    child[c] = -1;
-    parent[c] = P;
-    octant[c] = static_cast<uint8_t>(q);
-    level[c] = static_cast<int8_t>(d + 1);
-    for (int a = 0; a < 3; ++a) {
-        const double mn = bmin[3 * P + a], mx = bmax[3 * P + a];
-        const double mid = (mx + mn) / 2;
-        const bool low = (q >> a) & 1u;
-        bmin[3 * c + a] = low ? mn : mid;
-        bmax[3 * c + a] = low ? mid : mx;
-        grid[3 * c + a] = 2 * grid[3 * P + a] + (low ? 0 : 1);
-    }
-    for (int l = 0; l <= MAXD; ++l) anc[c * (MAXD + 1) + l] = l <= d ? anc[static_cast<int64_t>(P) * (MAXD + 1) + l] : (l == d + 1 ? static_cast<int32_t>(c) : -1);
+    write_child(c, P, q, d, lo, hi, start, count, child, parent, octant, level, bmin, bmax, grid, anc, pmn, pmx, pg, pa);
     flag[8ll * k + q] = (hi - lo) > thr ? 1 : 0;
     if (q == 0) child[P] = static_cast<int32_t>(child_off + 8ll * k);
 }
@@ -319,28 +345,17 @@ __global__ __launch_bounds__(256) void k_expand_lv(int d, uint32_t *lv, const in
     const unsigned q = static_cast<unsigned>(t & 7);
     const int32_t P = int_ids[i0 + k];
     const uint32_t s = start[P], e = s + count[P];
+    const int64_t c = child_off + 8ll * k + q;
+    double pmn[3], pmx[3];
+    int32_t pg[3], pa[MAXD + 1];
+    read_parent(c, P, d, bmin, bmax, grid, anc, pmn, pmx, pg, pa);
     const int shift = 3 * (MAXD - 1 - d);
     // child q ends where child q + 1 starts: the 8 children of a parent are 8
     // consecutive lanes (8-aligned), so one search per lane and a shuffle
     const uint32_t lo = digit_lower_bound(key, s, e, shift, q);
     const uint32_t nx = __shfl_down(lo, 1, 8);
     const uint32_t hi = q == 7 ? e : nx;
-    const int64_t c = child_off + 8ll * k + q;
-    start[c] = lo;
-    count[c] = hi - lo;
-    child[c] = -1;
-    parent[c] = P;
-    octant[c] = static_cast<uint8_t>(q);
-    level[c] = static_cast<int8_t>(d + 1);
-    for (int a = 0; a < 3; ++a) {
-        const double mn = bmin[3 * P + a], mx = bmax[3 * P + a];
-        const double mid = (mx + mn) / 2;
-        const bool low = (q >> a) & 1u;
-        bmin[3 * c + a] = low ? mn : mid;
-        bmax[3 * c + a] = low ? mid : mx;
-        grid[3 * c + a] = 2 * grid[3 * P + a] + (low ? 0 : 1);
-    }
-    for (int l = 0; l <= MAXD; ++l) anc[c * (MAXD + 1) + l] = l <= d ? anc[static_cast<int64_t>(P) * (MAXD + 1) + l] : (l == d + 1 ? static_cast<int32_t>(c) : -1);
+    write_child(c, P, q, d, lo, hi, start, count, child, parent, octant, level, bmin, bmax, grid, anc, pmn, pmx, pg, pa);
     flag[8ll * k + q] = (hi - lo) > thr ? 1 : 0;
     if (q == 0) child[P] = static_cast<int32_t>(child_off + 8ll * k);
 }
