@@ -477,6 +477,9 @@ int vipp_grav_step(vipp_grav_ctx *ctx, double dt, uint32_t nsteps) {
         need(c.uploaded, VIPP_GRAV_E_STATE, "step before upload");
         need(std::isfinite(dt), VIPP_GRAV_E_INVALID, "non-finite dt");
         VG_HIP(hipSetDevice(c.device));
+        // partial boxes of the RK4 stages serve the next tree build inside this
+        // call only: outside it the caller may have written x (device_state)
+        c.aabb_parts = 0;
         for (uint32_t s = 0; s < nsteps; ++s) {
             vg::rk4_init(c);
             try {
@@ -502,9 +505,11 @@ int vipp_grav_step(vipp_grav_ctx *ctx, double dt, uint32_t nsteps) {
                 (void)hipStreamSynchronize(c.stream);
                 c.evaluated = false;
                 c.t.ref_valid = false;
+                c.aabb_parts = 0;
                 throw;
             }
         }
+        c.aabb_parts = 0;
         VG_HIP(hipStreamSynchronize(c.stream));
         // the last evaluation saw the stage-3 positions, not the stepped ones:
         // tree and list queries after a step rebuild at the new positions

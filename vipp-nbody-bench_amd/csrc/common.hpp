@@ -74,6 +74,26 @@ struct DevBuf {
 __host__ __device__ inline int kidx_re(int n, int m) { return m == 0 ? n * n : n * n + 2 * m - 1; }
 __host__ __device__ inline int kidx_im(int n, int m) { return n * n + 2 * m; }
 
+// Block-wide minimum and maximum of each coordinate and a non-finite flag
+// over a block of 256 threads, stored as the 7 doubles of one partial box
+// (lo xyz, hi xyz, flag) at part.  Every thread of the block must call it.
+__device__ inline void aabb_block_store(const double *lo, const double *hi, double bad, double *part) {
+    __shared__ double s[7][256];
+    for (int a = 0; a < 3; ++a) {
+        s[a][threadIdx.x] = lo[a];
+        s[3 + a][threadIdx.x] = hi[a];
+    }
+    s[6][threadIdx.x] = bad;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < static_cast<unsigned>(w))
+            for (int a = 0; a < 7; ++a)
+                s[a][threadIdx.x] = a < 3 ? fmin(s[a][threadIdx.x], s[a][threadIdx.x + w]) : fmax(s[a][threadIdx.x], s[a][threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x < 7) part[threadIdx.x] = s[threadIdx.x][0];
+}
+
 // M2L by mirror symmetry.  The 189 well-separated (parent neighbour d, source
 // octant q) pairs of a target of octant 0 are walked in one fixed order; for a
 // target of octant qt the geometry is octant 0's mirrored in the axes where qt

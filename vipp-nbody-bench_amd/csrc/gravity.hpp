@@ -117,6 +117,9 @@ struct Ctx {
 
     // tree build scratch
     DevBuf<double> aabb_part;   // per-block partial boxes
+    // partial boxes of the current x left by the RK4 stage that wrote it (0:
+    // none); trusted only inside one step call, where nothing else writes x
+    int aabb_parts = 0;
     DevBuf<double> box;         // 6 doubles + flag
     DevBuf<uint64_t> key[2];
     DevBuf<uint32_t> val[2];

@@ -26,26 +26,47 @@ __global__ __launch_bounds__(256) void k_rk4_init(int64_t n3, const double *__re
 }
 
 // accumulate with weight cw, then take the next Euler state with step h
-// (last stage: copy the accumulated state out instead).
+// (last stage: copy the accumulated state out instead).  Grid-stride over the
+// 3n scalars; each block also leaves the partial bounding box of the positions
+// it wrote (7 doubles at part, the layout of k_aabb), so the next tree build
+// needs no pass of its own over x.
 template <class A>
 __global__ __launch_bounds__(256) void k_rk4_stage(int64_t n3, double cw, double h, int last, const A *__restrict__ a, double *__restrict__ x,
                                                    double *__restrict__ v, const double *__restrict__ x0, const double *__restrict__ v0,
-                                                   double *__restrict__ xo, double *__restrict__ vo) {
-    const int64_t i = blockIdx.x * 256ll + threadIdx.x;
-    if (i >= n3) return;
-    const double ai = static_cast<double>(a[i]);
-    const double vi = v[i];
-    const double xon = xo[i] + cw * vi;
-    const double von = vo[i] + cw * ai;
-    if (last) {
-        x[i] = xon;
-        v[i] = von;
-    } else {
-        xo[i] = xon;
-        vo[i] = von;
-        x[i] = x0[i] + vi * h;
-        v[i] = v0[i] + ai * h;
+                                                   double *__restrict__ xo, double *__restrict__ vo, double *__restrict__ part) {
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    double bad = 0;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * 256;
+    const int s3 = static_cast<int>(stride % 3);
+    int64_t i = blockIdx.x * 256ll + threadIdx.x;
+    int comp = static_cast<int>(i % 3);
+    for (; i < n3; i += stride) {
+        const double ai = static_cast<double>(a[i]);
+        const double vi = v[i];
+        const double xon = xo[i] + cw * vi;
+        const double von = vo[i] + cw * ai;
+        double xn;
+        if (last) {
+            xn = xon;
+            v[i] = von;
+        } else {
+            xo[i] = xon;
+            vo[i] = von;
+            xn = x0[i] + vi * h;
+            v[i] = v0[i] + ai * h;
+        }
+        x[i] = xn;
+        if (!isfinite(xn)) bad = 1;
+        lo[0] = comp == 0 ? fmin(lo[0], xn) : lo[0];
+        lo[1] = comp == 1 ? fmin(lo[1], xn) : lo[1];
+        lo[2] = comp == 2 ? fmin(lo[2], xn) : lo[2];
+        hi[0] = comp == 0 ? fmax(hi[0], xn) : hi[0];
+        hi[1] = comp == 1 ? fmax(hi[1], xn) : hi[1];
+        hi[2] = comp == 2 ? fmax(hi[2], xn) : hi[2];
+        comp += s3;
+        if (comp >= 3) comp -= 3;
     }
+    aabb_block_store(lo, hi, bad, part + 7ll * blockIdx.x);
 }
 
 template <class R, class A>
@@ -126,11 +147,15 @@ void rk4_stage(Ctx &c, int stage, double dt) {
     const double cw = (stage == 0 || stage == 3) ? dt / 6 : dt / 3;
     const double h = stage == 2 ? dt : dt / 2;
     const int last = stage == 3;
+    const int nb = static_cast<int>(std::min<int64_t>(1024, std::max<int64_t>(1, ceil_div(n3, 256))));
+    c.aabb_part.ensure(7 * static_cast<size_t>(nb));
+    c.aabb_parts = 0;
     if (c.prec == 0)
-        k_rk4_stage<double><<<ceil_div(n3, 256), 256, 0, c.stream>>>(n3, cw, h, last, c.acc64, c.x, c.v, c.x0, c.v0, c.xo, c.vo);
+        k_rk4_stage<double><<<nb, 256, 0, c.stream>>>(n3, cw, h, last, c.acc64, c.x, c.v, c.x0, c.v0, c.xo, c.vo, c.aabb_part);
     else
-        k_rk4_stage<float><<<ceil_div(n3, 256), 256, 0, c.stream>>>(n3, cw, h, last, c.acc32, c.x, c.v, c.x0, c.v0, c.xo, c.vo);
+        k_rk4_stage<float><<<nb, 256, 0, c.stream>>>(n3, cw, h, last, c.acc32, c.x, c.v, c.x0, c.v0, c.xo, c.vo, c.aabb_part);
     VG_LAUNCH_CHECK();
+    c.aabb_parts = nb;
 }
 
 void scatter_accel(Ctx &c, bool potential) {

@@ -23,7 +23,6 @@ namespace {
 constexpr double PADDING = 1e-5;  // Problem::OCTREE_WIDTH_PADDING (problem.hpp:167), (double)1E-5L == 1e-5
 
 __global__ __launch_bounds__(256) void k_aabb(const double *__restrict__ x, int64_t n, double *__restrict__ part) {
-    __shared__ double s[6][256];
     double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     double bad = 0;
     for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * 256) {
@@ -34,25 +33,7 @@ __global__ __launch_bounds__(256) void k_aabb(const double *__restrict__ x, int6
             hi[a] = fmax(hi[a], v);
         }
     }
-    for (int a = 0; a < 3; ++a) {
-        s[a][threadIdx.x] = lo[a];
-        s[3 + a][threadIdx.x] = hi[a];
-    }
-    __shared__ double sb[256];
-    sb[threadIdx.x] = bad;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < static_cast<unsigned>(w)) {
-            for (int a = 0; a < 3; ++a) {
-                s[a][threadIdx.x] = fmin(s[a][threadIdx.x], s[a][threadIdx.x + w]);
-                s[3 + a][threadIdx.x] = fmax(s[3 + a][threadIdx.x], s[3 + a][threadIdx.x + w]);
-            }
-            sb[threadIdx.x] = fmax(sb[threadIdx.x], sb[threadIdx.x + w]);
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x < 6) part[blockIdx.x * 7 + threadIdx.x] = s[threadIdx.x][0];
-    if (threadIdx.x == 6) part[blockIdx.x * 7 + 6] = sb[0];
+    aabb_block_store(lo, hi, bad, part + 7ll * blockIdx.x);
 }
 
 // final reduction; box = padded root bounds (fmm.hpp:403-405), box[6] = non-finite flag
@@ -919,12 +900,18 @@ void tree_build(Ctx &c) {
     TreeDev &t = c.t;
     const int64_t n = c.n;
     // 1. root box
-    const int nb = static_cast<int>(std::min<int64_t>(1024, std::max<int64_t>(1, (n + 255) / 256)));
-    c.aabb_part.ensure(7 * static_cast<size_t>(nb));
+    //    (the RK4 stage that wrote x may have left its partial boxes)
     c.box.ensure(8);
     c.scal.ensure(16);
-    k_aabb<<<nb, 256, 0, st>>>(c.x, n, c.aabb_part);
-    k_aabb_final<<<1, 256, 0, st>>>(c.aabb_part, nb, c.box);
+    if (c.aabb_parts > 0) {
+        k_aabb_final<<<1, 256, 0, st>>>(c.aabb_part, c.aabb_parts, c.box);
+    } else {
+        const int nb = static_cast<int>(std::min<int64_t>(1024, std::max<int64_t>(1, (n + 255) / 256)));
+        c.aabb_part.ensure(7 * static_cast<size_t>(nb));
+        k_aabb<<<nb, 256, 0, st>>>(c.x, n, c.aabb_part);
+        k_aabb_final<<<1, 256, 0, st>>>(c.aabb_part, nb, c.box);
+    }
+    c.aabb_parts = 0;
     VG_LAUNCH_CHECK();
     // 2. path codes and a stable sort of the digits of levels 1..SHALLOW
     //    (bits below the deepest level do not matter: leaves are re-sorted by
