@@ -123,20 +123,24 @@ __device__ inline void row_of(int r, int &j, int &k, int &part) {
 // op block layout: [M2M 0..7][L2L 8..15][M2L 16..16+342], each K*K; M2L
 // blocks row-major [row][col], M2M and L2L blocks column-major [col][row] (their
 // kernels give each lane one output row, so lanes then read consecutive words).
-// One block per (operator, row r); thread t takes the column pair (n, m >= 0)
-// number t, i.e. the real-packed columns of degree n, order m.
+// One block per (operator, OPS_ROWS rows), 128 threads per row r; thread t of
+// a row takes the column pair (n, m >= 0) number t, i.e. the real-packed
+// columns of degree n, order m.
+constexpr int OPS_ROWS = 4;
 template <class R>
-__global__ __launch_bounds__(128) void k_ops(const double *__restrict__ harm, int p, int K, R *__restrict__ op) {
+__global__ __launch_bounds__(128 * OPS_ROWS) void k_ops(const double *__restrict__ harm, int p, int K, R *__restrict__ op) {
     const int blk = blockIdx.x;  // operator index
-    const int r = blockIdx.y;
-    int j, k, part;
-    row_of(r, j, k, part);
+    const int r = blockIdx.y * OPS_ROWS + static_cast<int>(threadIdx.x >> 7);
+    const int t = threadIdx.x & 127;
     const bool cmaj = blk < 16;
     R *dst = cmaj ? op + static_cast<size_t>(blk) * K * K + r : op + (static_cast<size_t>(blk) * K + r) * K;
     const size_t cs = cmaj ? static_cast<size_t>(K) : 1;  // column stride
-    for (int c = threadIdx.x; c < K; c += 128) dst[c * cs] = R(0);
+    if (r < K)
+        for (int c = t; c < K; c += 128) dst[c * cs] = R(0);
     __syncthreads();
-    const int t = threadIdx.x;
+    if (r >= K) return;
+    int j, k, part;
+    row_of(r, j, k, part);
     if (t >= (p + 1) * (p + 2) / 2) return;
     int n = 0;
     while ((n + 1) * (n + 2) / 2 <= t) ++n;
@@ -1010,18 +1014,18 @@ void operators_build(Ctx &c) {
     VG_HIP(hipMemcpyAsync(c.opvec.p, vec.data(), vec.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
     c.harm.ensure(static_cast<size_t>(nvec) * HSTRIDE);
     k_harmonics<<<nvec, 64, 0, c.stream>>>(c.opvec, nvec, 2 * c.p, c.harm);
-    dim3 grid(nvec, c.K);
+    dim3 grid(nvec, ceil_div(c.K, OPS_ROWS));
     if (c.prec == 0) {
         c.op64.ensure(static_cast<size_t>(nvec) * c.K * c.K);
-        k_ops<double><<<grid, 128, 0, c.stream>>>(c.harm, c.p, c.K, c.op64);
+        k_ops<double><<<grid, 128 * OPS_ROWS, 0, c.stream>>>(c.harm, c.p, c.K, c.op64);
     } else if (c.prec == 2) {
         c.op64.ensure(static_cast<size_t>(nvec) * c.K * c.K);
-        k_ops<double><<<grid, 128, 0, c.stream>>>(c.harm, c.p, c.K, c.op64);
+        k_ops<double><<<grid, 128 * OPS_ROWS, 0, c.stream>>>(c.harm, c.p, c.K, c.op64);
         c.op32.ensure(static_cast<size_t>(nvec) * c.K * c.K);
         m2l_split_operators(c, nvec);
     } else {
         c.op32.ensure(static_cast<size_t>(nvec) * c.K * c.K);
-        k_ops<float><<<grid, 128, 0, c.stream>>>(c.harm, c.p, c.K, c.op32);
+        k_ops<float><<<grid, 128 * OPS_ROWS, 0, c.stream>>>(c.harm, c.p, c.K, c.op32);
     }
     VG_LAUNCH_CHECK();
 }

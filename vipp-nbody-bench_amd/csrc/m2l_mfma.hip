@@ -35,29 +35,32 @@ __host__ __device__ constexpr int mt_of(int K) { return (K + 31) / 32; }
 __host__ __device__ constexpr int ks_of(int K) { return (K + 15) / 16; }
 
 // FP64 operators (row-major K x K per offset) -> BF16 hi/lo A fragments of
-// the M2L blocks: frag[o][mt][h][ks][lane][e], so
-// that any run of consecutive m-tiles (hi and lo) is one contiguous range.
-__global__ __launch_bounds__(256) void k_split_ops(const double *__restrict__ op64, int nblk, int K, float *__restrict__ op32, __bf16 *__restrict__ frag,
-                                                   int MT, int NKS) {
-    const int64_t i = blockIdx.x * 256ll + threadIdx.x;
-    const int64_t kk = static_cast<int64_t>(K) * K;
-    if (op32 && i < nblk * kk) op32[i] = static_cast<float>(op64[i]);  // FP32 copies only when asked for
-    const int64_t per = static_cast<int64_t>(MT) * NKS * 64;  // lanes per operator
-    if (i >= 343 * per) return;
-    const int o = static_cast<int>(i / per);
-    const int rem = static_cast<int>(i % per);
-    const int mt = rem / (NKS * 64), ks = (rem / 64) % NKS, l = rem % 64;
-    const double *src = op64 + static_cast<int64_t>(16 + o) * kk;
-    const int row = mt * 32 + (l & 31);
-    const int64_t vec = (static_cast<int64_t>(o) * MT + mt) * 2 * NKS * 64 + ks * 64 + l;
-    __bf16 *hi = frag + vec * 8;
-    __bf16 *lo = frag + (vec + NKS * 64) * 8;
-    for (int e = 0; e < 8; ++e) {
-        const int col = ks * 16 + (l >> 5) * 8 + e;
-        const double x = (row < K && col < K) ? src[static_cast<int64_t>(row) * K + col] : 0.0;
-        const __bf16 h = static_cast<__bf16>(static_cast<float>(x));
-        hi[e] = h;
-        lo[e] = static_cast<__bf16>(static_cast<float>(x - static_cast<double>(static_cast<float>(h))));
+// the M2L blocks: frag[o][mt][h][ks][lane][e], so that any run of consecutive
+// m-tiles (hi and lo) is one contiguous range.  One block per (offset, m-tile):
+// the tile's 32 rows are read row by row into LDS (coalesced), then each thread
+// writes whole 8-coefficient fragment vectors.
+__global__ __launch_bounds__(256) void k_split_ops(const double *__restrict__ op64, int K, __bf16 *__restrict__ frag, int MT, int NKS) {
+    extern __shared__ double tile[];  // 32 rows x (16 NKS + 1)
+    const int o = blockIdx.x / MT, mt = blockIdx.x % MT;
+    const int kc = 16 * NKS, W = kc + 1;
+    const double *src = op64 + static_cast<int64_t>(16 + o) * K * K;
+    for (int i = threadIdx.x; i < 32 * kc; i += 256) {
+        const int r = i / kc, col = i % kc, row = mt * 32 + r;
+        tile[r * W + col] = (row < K && col < K) ? src[static_cast<int64_t>(row) * K + col] : 0.0;
+    }
+    __syncthreads();
+    const int64_t base = (static_cast<int64_t>(o) * MT + mt) * 2 * NKS * 64;  // in 8-coefficient vectors
+    for (int v = threadIdx.x; v < 2 * NKS * 64; v += 256) {
+        const int h = v / (NKS * 64), ks = (v / 64) % NKS, l = v % 64;
+        const int r = l & 31, c0 = ks * 16 + (l >> 5) * 8;
+        bf16x8 out;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const double x = tile[r * W + c0 + e];
+            const __bf16 hv = static_cast<__bf16>(static_cast<float>(x));
+            out[e] = h ? static_cast<__bf16>(static_cast<float>(x - static_cast<double>(static_cast<float>(hv)))) : hv;
+        }
+        reinterpret_cast<bf16x8 *>(frag)[base + v] = out;
     }
 }
 
@@ -623,8 +626,8 @@ void m2l_split_operators(Ctx &c, int nvec) {
     const int64_t per = static_cast<int64_t>(MT) * NKS * 64;
     c.opsplit.ensure(static_cast<size_t>(343 * 2 * per * 8));
     // split mode runs M2M, M2L and L2L from the fragments: no FP32 copies
-    const int64_t n = 343 * per;
-    k_split_ops<<<ceil_div(n, 256), 256, 0, c.stream>>>(c.op64, nvec, c.K, nullptr, reinterpret_cast<__bf16 *>(c.opsplit.p), MT, NKS);
+    (void)nvec;
+    k_split_ops<<<343 * MT, 256, 32 * (16 * NKS + 1) * sizeof(double), c.stream>>>(c.op64, c.K, reinterpret_cast<__bf16 *>(c.opsplit.p), MT, NKS);
     c.opsplit_ml.ensure(static_cast<size_t>(16 * 3 * per * 8));
     k_split_ops_ml<<<ceil_div(16 * per, 256), 256, 0, c.stream>>>(c.op64, c.K, reinterpret_cast<__bf16 *>(c.opsplit_ml.p), MT, NKS);
     VG_LAUNCH_CHECK();
