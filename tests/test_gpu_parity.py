@@ -523,6 +523,23 @@ def test_deterministic(vg, orc):
     assert np.array_equal(outs[0], outs[1])
 
 
+@pytest.mark.parametrize("mode", ["f64", "split"])
+def test_stage_boxes_equal_fresh_boxes(vg, orc, mode):
+    """Inside one step call each tree takes its root box from the partial
+    boxes the previous RK4 stage left; a call starts from a box computed from
+    x.  Minimum and maximum are exact, so one call of two steps and two calls
+    of one step give the same state bit for bit."""
+    x, v, _ = orc.problem_initialize(20000, 12, 1e-3)
+    outs = []
+    for calls in ((2,), (1, 1)):
+        with vg.Gravity(len(x), 64, 6, mode, eps=1e-3) as gr:
+            gr.upload(x, v)
+            for k in calls:
+                gr.step(1e-3, k)
+            outs.append(gr.download())
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+
+
 # ------------------------------------------------------------- every order --
 @pytest.mark.parametrize("p", list(range(1, 13)))
 def test_every_order(vg, orc, p):
