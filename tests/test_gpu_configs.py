@@ -16,7 +16,9 @@ config 4: the two-Plummer collision.  At its full N = 2^24, tree and lists
   p = 12 FMM (median 1e-5, the BF16 hi/lo M2L) and the FP64 direct sum
   (median 1e-5), and the total energy drifts less than 1e-5 over 10 steps.
 Energy: total energy of a 16k Plummer sphere under 20 split-mode RK4 steps
-  drifts less than 1e-5 (relative)."""
+  drifts less than 1e-5 (relative).
+Scale: one split-mode p = 8 evaluation of 2^26 uniform points (16 times the
+  headline size) against the FP64 direct sum on a sample."""
 import numpy as np
 import pytest
 
@@ -150,3 +152,21 @@ def test_config4_collision_split_vs_oracle(vg, orc, workloads):
     drift = abs(e1 - e0) / abs(e0)
     print(f"collision 2^16: median vs FP64 FMM {np.median(rel):.2e}, vs direct {np.median(rel_d):.2e}, drift {drift:.2e}")
     assert drift < 1e-5, drift
+
+
+def test_large_n_uniform_split(vg):
+    """2^26 points, 16 times the headline size (index arithmetic of the sort,
+    the tree, the lists, P2P work items and the particle arrays at that scale):
+    one split-mode evaluation against the FP64 direct sum on a sample."""
+    n = 1 << 26
+    x = np.random.default_rng(21).random((n, 3))
+    with vg.Gravity(n, 64, 8, "split", eps=1e-4, G=1.0, mass=1.0 / n) as g:
+        g.upload(x, np.zeros_like(x))
+        a = g.accelerations()
+        idx = np.sort(np.random.default_rng(22).choice(n, 2048, replace=False))
+        d = g.direct(idx)
+        info = g.counts()
+    rel = np.linalg.norm(a[idx] - d, axis=1) / np.linalg.norm(d, axis=1)
+    print(f"2^26 uniform: cells {info['cells']}, leaves {info['leaves']}, depth {info['depth']}, median rel {np.median(rel):.2e}")
+    assert np.isfinite(a).all()
+    assert np.median(rel) < 1e-5 and np.percentile(rel, 99) < 1e-3
