@@ -841,14 +841,21 @@ template <class R, int TB, int RPT, int RG = 1>
 void m2l_plain_launch(Ctx &c, const int32_t *parents, int np, unsigned grid) {
     constexpr int KC = sizeof(R) == 4 ? 16 : 8;
     const size_t lds = (static_cast<size_t>(c.K) * (TB + 16 / sizeof(R)) + 2 * KC * (32 * RPT + 1)) * sizeof(R);
-    // dynamic LDS above 64 KiB is allowed per kernel, here for the largest K of this RPT
-    static const bool attr = [] {
-        const size_t cap = (static_cast<size_t>(32 * RPT) * (TB + 16 / sizeof(R)) + 2 * KC * (32 * RPT + 1)) * sizeof(R);
-        VG_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_m2l_plain<R, TB, RPT, RG>), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   static_cast<int>(cap)));
-        return true;
-    }();
-    (void)attr;
+    // dynamic LDS above 64 KiB is allowed per kernel and device, here for the
+    // largest K of this RPT (set once per device, under a lock: contexts of
+    // different host threads may launch concurrently)
+    static std::mutex mu;
+    static uint64_t done = 0;  // bit d: set on device d
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        const uint64_t bit = c.device < 64 ? (1ull << c.device) : 0;
+        if (!bit || !(done & bit)) {
+            const size_t cap = (static_cast<size_t>(32 * RPT) * (TB + 16 / sizeof(R)) + 2 * KC * (32 * RPT + 1)) * sizeof(R);
+            VG_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_m2l_plain<R, TB, RPT, RG>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       static_cast<int>(cap)));
+            done |= bit;
+        }
+    }
     const TreeDev &t = c.t;
     k_m2l_plain<R, TB, RPT, RG><<<grid, 256 * RG, lds, c.stream>>>(parents, np, t.child, t.count, t.nbr27, Bufs<R>::M(c), Bufs<R>::L(c), Bufs<R>::op(c), c.K, c.ldk);
     VG_LAUNCH_CHECK();
