@@ -523,6 +523,31 @@ def test_deterministic(vg, orc):
     assert np.array_equal(outs[0], outs[1])
 
 
+@pytest.mark.parametrize("mode", ["f64", "f32", "split"])
+@pytest.mark.parametrize("n", [1, 2, 5, 100])
+def test_tiny_inputs(vg, orc, mode, n):
+    """Fewer points than a leaf holds: the root is the only leaf, the far field
+    is empty and the forces are the direct sum.  One point feels nothing.  A
+    step and the energy work at these sizes too."""
+    x = np.random.default_rng(n).random((n, 3))
+    v = np.zeros_like(x)
+    with vg.Gravity(n, 64, 4, mode, eps=1e-2) as gr:
+        gr.upload(x, v)
+        a = gr.accelerations()
+        d = gr.direct(np.arange(n))
+        gr.step(1e-3, 2)
+        x1, v1 = gr.download()
+        e = gr.energy()
+        assert gr.counts()["cells"] == 1
+    ref = orc.direct(x, eps=1e-2)
+    tol = 1e-12 if mode == "f64" else 2e-6
+    assert np.allclose(d, ref, rtol=1e-13, atol=0)
+    assert np.abs(a - ref).max() <= tol * max(np.abs(ref).max(), 1e-300)
+    if n == 1:
+        assert not a.any() and np.array_equal(x1, x)
+    assert np.isfinite(x1).all() and np.isfinite(v1).all() and np.isfinite(e).all()
+
+
 @pytest.mark.parametrize("mode", ["f64", "split"])
 def test_stage_boxes_equal_fresh_boxes(vg, orc, mode):
     """Inside one step call each tree takes its root box from the partial
