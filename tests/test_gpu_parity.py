@@ -548,6 +548,38 @@ def test_tiny_inputs(vg, orc, mode, n):
     assert np.isfinite(x1).all() and np.isfinite(v1).all() and np.isfinite(e).all()
 
 
+@pytest.mark.parametrize("shape", ["plane", "line"])
+def test_degenerate_geometry(vg, orc, shape):
+    """Points in a plane (z = 0) or on a line: the root box is 2e-5 thick in
+    the flat axes (the reference's padding), so the cells are extremely flat
+    and the offsets of the operators nearly degenerate.  Tree and lists stay
+    bit-exact, FP64 forces match the oracle's FMM to 1e-11, and split mode
+    matches the FP64 direct sum."""
+    rng = np.random.default_rng(31)
+    n = 6000
+    x = np.zeros((n, 3))
+    x[:, 0] = rng.normal(size=n)
+    if shape == "plane":
+        x[:, 1] = rng.normal(size=n)
+    with vg.Gravity(n, 16, 6, "f64", eps=1e-3) as gr:
+        gr.upload(x)
+        a64 = gr.accelerations()
+        t, l = gr.tree(), gr.lists()
+    rt, rl = orc.tree_and_lists(x, 16)
+    for f in ("parent", "level", "child_base", "pt_off", "pts"):
+        assert np.array_equal(getattr(t, f), getattr(rt, f)), f
+    for f in ("ap_off", "ap", "ni_off", "ni", "nb_off", "nb"):
+        assert np.array_equal(getattr(l, f), getattr(rl, f)), f
+    ref = orc.fmm(x, 16, 6, eps=1e-3)
+    assert np.abs(a64 - ref).max() <= 1e-11 * np.abs(ref).max()
+    with vg.Gravity(n, 16, 6, "split", eps=1e-3) as gr:
+        gr.upload(x)
+        a = gr.accelerations()
+        d = gr.direct(np.arange(0, n, 7))
+    rel = np.linalg.norm(a[::7] - d, axis=1) / np.linalg.norm(d, axis=1)
+    assert np.median(rel) < 1e-3, np.median(rel)
+
+
 @pytest.mark.parametrize("mode", ["f64", "split"])
 def test_stage_boxes_equal_fresh_boxes(vg, orc, mode):
     """Inside one step call each tree takes its root box from the partial
