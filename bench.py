@@ -337,7 +337,7 @@ def learned_comparison(vgm, workloads, args):
     trained on the device on pairs the FMM generates from three other seeds of
     the workload (never the evaluated one).  The fallback bound is swept over
     the fraction of leaves sent to order 12; errors are stated against the
-    FP64 direct sum and against the FP64 order-12 FMM (the judge of the
+    FP64 direct sum and against the FP64 order-12 FMM (the reference of the
     closure), and the speed-up is stated where the closure's error reaches
     the plain FMM's."""
     import closure
@@ -352,7 +352,7 @@ def learned_comparison(vgm, workloads, args):
     idx = np.sort(np.random.default_rng(1).choice(n, size=min(n, args.learned_err_sample), replace=False))
     with vgm.Gravity(n, wdef["s"], 12, "f64", eps=wdef["eps"], G=1.0, mass=1.0 / n) as g:
         g.upload(x, v, masses)
-        judge = g.accelerations()[idx]
+        ref12 = g.accelerations()[idx]
         d = g.direct(idx)
 
     def run(p, mode, cl=None):
@@ -363,7 +363,7 @@ def learned_comparison(vgm, workloads, args):
                 g.set_closure(*cl)
             a = g.accelerations()[idx]
             out["fallback_leaves"] = g.counts()["fallback_leaves"]
-        out["err_vs_fp64_p12"] = rel_err(a, judge)
+        out["err_vs_fp64_p12"] = rel_err(a, ref12)
         return out
 
     plain = run(12, "f32")
@@ -377,7 +377,7 @@ def learned_comparison(vgm, workloads, args):
         r.update({"fallback_fraction_target": frac, "bound": b})
         sweep.append(r)
     # equal error: the smallest swept fallback whose median error (against the
-    # FP64 direct sum, and against the FP64 p = 12 judge) is at or below plain's
+    # FP64 direct sum, and against the FP64 p = 12 FMM) is at or below plain's
     def at_equal(key):
         target = plain[key]["median"]
         for r in sweep:
