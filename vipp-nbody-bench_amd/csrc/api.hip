@@ -78,9 +78,11 @@ void join_p2p(Ctx &c) {
 // Untimed split-mode evaluations run the lists and P2P (VALU-bound, about
 // half the evaluation) on a second stream once the tree exists, so that the
 // operators and the upward pass (which need only the tree), and M2L
-// (MFMA-bound, after the lists), run beside them; L2P waits for P2P. The FP32
-// and FP64 modes, whose M2L is VALU-bound too, and timed evaluations keep one
-// stream (the events of a phase then bracket its own kernels only).
+// (MFMA-bound, after the lists), run beside them; L2P waits for P2P. The P2P
+// plan is enqueued with the lists and its item counts are read back while the
+// host enqueues the operators and the upward pass. The FP32 and FP64 modes,
+// whose M2L is VALU-bound too, and timed evaluations keep one stream (the
+// events of a phase then bracket its own kernels only).
 static void evaluate_overlapped(Ctx &c, bool potential) {
     VG_HIP(hipEventRecord(c.ev_fork, c.stream));
     VG_HIP(hipStreamWaitEvent(c.stream2, c.ev_fork, 0));
@@ -95,11 +97,15 @@ static void evaluate_overlapped(Ctx &c, bool potential) {
             lists_build(c);
         }
         VG_HIP(hipEventRecord(c.ev_lists, c.stream2));
-        operators_build(c);
-        fmm_upward(c);  // enqueued first: the GPU has work while the host waits for the P2P plan
         {
             Swap on_stream2(c);
-            fmm_p2p(c, potential);
+            fmm_p2p_plan(c);
+        }
+        operators_build(c);
+        fmm_upward(c);  // enqueued while the P2P plan's item counts come back
+        {
+            Swap on_stream2(c);
+            fmm_p2p_run(c, potential);
         }
         VG_HIP(hipEventRecord(c.ev_join, c.stream2));
     } catch (...) {
@@ -394,6 +400,7 @@ int vipp_grav_create(const vipp_grav_config *cfg, vipp_grav_ctx **out) {
             VG_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
             VG_HIP(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
             VG_HIP(hipEventCreateWithFlags(&c.ev_lists, hipEventDisableTiming));
+            VG_HIP(hipEventCreateWithFlags(&c.ev_plan, hipEventDisableTiming));
             VG_HIP(hipHostMalloc(&c.pinned, vg::Ctx::PINNED_BYTES, hipHostMallocDefault));
             for (auto &e : c.ev) VG_HIP(hipEventCreate(&e));
             vg::fmm_constants_init();
@@ -425,6 +432,7 @@ int vipp_grav_destroy(vipp_grav_ctx *ctx) {
     if (ctx->c.ev_fork) (void)hipEventDestroy(ctx->c.ev_fork);
     if (ctx->c.ev_join) (void)hipEventDestroy(ctx->c.ev_join);
     if (ctx->c.ev_lists) (void)hipEventDestroy(ctx->c.ev_lists);
+    if (ctx->c.ev_plan) (void)hipEventDestroy(ctx->c.ev_plan);
     if (ctx->c.stream) (void)hipStreamDestroy(ctx->c.stream);
     if (ctx->c.stream2) (void)hipStreamDestroy(ctx->c.stream2);
     if (ctx->c.pinned) (void)hipHostFree(ctx->c.pinned);

@@ -62,11 +62,27 @@ constexpr int HSTRIDE = 2 * ((AL + 1) * (AL + 2) / 2) + 2;  // per offset: Y tab
 // Y_l^m(direction) for l <= lmax and |v| of each offset vector.
 // One thread per (offset, m): P_l^m by the upward recurrence in l, with
 // P_m^m = (-1)^m (2m-1)!! sin^m, and sqrt((l-m)!/(l+m)!) carried along l.
-__global__ __launch_bounds__(64) void k_harmonics(const double *__restrict__ vec, int nvec, int lmax, double *__restrict__ out) {
+// Offset vectors in units of a level's x width, the cell aspect ratio being
+// (1, ry, rz): 0..7 M2M (child centre - parent centre of child octant o),
+// 8..15 L2L (their negatives), 16 + o M2L (source - target cell, o in base 7
+// holding (dx + 3, dy + 3, dz + 3)).
+__global__ __launch_bounds__(64) void k_harmonics(int nvec, double ry, double rz, int lmax, double *__restrict__ out) {
     const int o = blockIdx.x;
     const int m = threadIdx.x;
     if (o >= nvec || m > lmax) return;
-    const double vx = vec[3 * o], vy = vec[3 * o + 1], vz = vec[3 * o + 2];
+    double vx, vy, vz;
+    if (o < 16) {
+        const int q = o & 7;
+        const double cx = (q & 1) ? -0.25 : 0.25, cy = ((q >> 1) & 1) ? -0.25 * ry : 0.25 * ry, cz = ((q >> 2) & 1) ? -0.25 * rz : 0.25 * rz;
+        vx = o < 8 ? cx : -cx;
+        vy = o < 8 ? cy : -cy;
+        vz = o < 8 ? cz : -cz;
+    } else {
+        const int e = o - 16;
+        vx = e % 7 - 3;
+        vy = ((e / 7) % 7 - 3) * ry;
+        vz = (e / 49 - 3) * rz;
+    }
     double *tab = out + static_cast<size_t>(o) * HSTRIDE;
     const double rho = sqrt(vx * vx + vy * vy + vz * vz);
     if (m == 0) {
@@ -999,28 +1015,11 @@ void operators_build(Ctx &c) {
     const TreeDev &t = c.t;
     const double wx = t.box[3] - t.box[0], wy = t.box[4] - t.box[1], wz = t.box[5] - t.box[2];
     const double ry = wx > 0 ? wy / wx : 1.0, rz = wx > 0 ? wz / wx : 1.0;
-    // offset vectors in units of the x width of the level: 8 M2M (child - parent), 8 L2L (parent - child), 343 M2L (source - target)
-    std::vector<double> &vec = c.opvec_host;  // kept alive with the context: the upload below is asynchronous
-    vec.assign(3 * (16 + 343), 0.0);
-    for (int q = 0; q < 8; ++q) {
-        const double vx = (q & 1) ? -0.25 : 0.25, vy = ((q >> 1) & 1) ? -0.25 * ry : 0.25 * ry, vz = ((q >> 2) & 1) ? -0.25 * rz : 0.25 * rz;
-        vec[3 * q] = vx;
-        vec[3 * q + 1] = vy;
-        vec[3 * q + 2] = vz;
-        vec[3 * (8 + q)] = -vx;
-        vec[3 * (8 + q) + 1] = -vy;
-        vec[3 * (8 + q) + 2] = -vz;
-    }
-    for (int o = 0; o < 343; ++o) {
-        vec[3 * (16 + o)] = o % 7 - 3;
-        vec[3 * (16 + o) + 1] = ((o / 7) % 7 - 3) * ry;
-        vec[3 * (16 + o) + 2] = (o / 49 - 3) * rz;
-    }
+    // 8 M2M (child - parent), 8 L2L (parent - child) and 343 M2L (source -
+    // target) offsets, formed on the device from the aspect ratio
     const int nvec = 16 + 343;
-    c.opvec.ensure(vec.size());
-    VG_HIP(hipMemcpyAsync(c.opvec.p, vec.data(), vec.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
     c.harm.ensure(static_cast<size_t>(nvec) * HSTRIDE);
-    k_harmonics<<<nvec, 64, 0, c.stream>>>(c.opvec, nvec, 2 * c.p, c.harm);
+    k_harmonics<<<nvec, 64, 0, c.stream>>>(nvec, ry, rz, 2 * c.p, c.harm);
     dim3 grid(nvec, ceil_div(c.K, OPS_ROWS));
     if (c.prec == 0) {
         c.op64.ensure(static_cast<size_t>(nvec) * c.K * c.K);

@@ -96,9 +96,10 @@ struct Ctx {
     hipStream_t stream = nullptr;
     // split mode: the lists and P2P run on stream2 beside the upward pass and
     // M2L of the same evaluation (they share no buffer); M2L waits for the
-    // lists (ev_lists), L2P for P2P (join_p2p)
+    // lists (ev_lists), the scatter for P2P (join_p2p)
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_lists = nullptr;
+    hipEvent_t ev_plan = nullptr;  // the P2P plan scalars are in the pinned words
     bool p2p_pending = false;
     // pinned host words for the scalars read back during an evaluation (tree
     // level sizes, root box, P2P plan): a pinned copy returns sooner
@@ -147,8 +148,6 @@ struct Ctx {
     DevBuf<float> ml_part;      // split M2M: the 8 octant products of one level before their sum
     DevBuf<uint16_t> msplit;    // multipoles as BF16 hi / lo halves (per cell ldk hi, then ldk lo)
     DevBuf<double> consts;      // A_n^m and recurrence constants (device copy)
-    DevBuf<double> opvec;       // operator offset vectors (device copy)
-    std::vector<double> opvec_host;
 
     // ref-numbering scratch
     DevBuf<uint32_t> slist;
@@ -174,7 +173,9 @@ void ref_numbering(Ctx &c);              // reference cell ids (for export only)
 void operators_build(Ctx &c);            // M2M / L2L / M2L operators for the current root box
 void fmm_upward(Ctx &c);                 // P2M + M2M
 void fmm_m2l(Ctx &c);                    // M2L (plain VALU or split MFMA by mode)
-void fmm_p2p(Ctx &c, bool potential);    // near field into the sorted accelerations
+void fmm_p2p(Ctx &c, bool potential);    // near field into the sorted accelerations (plan, then run)
+void fmm_p2p_plan(Ctx &c);               // P2P work items; the item counts are read back without waiting
+void fmm_p2p_run(Ctx &c, bool potential);  // waits for the item counts, then the P2P kernels
 void fmm_downward(Ctx &c, bool potential);  // L2L + L2P
 void join_p2p(Ctx &c);                       // ctx.stream waits for a P2P launched on ctx.stream2
 void scatter_accel(Ctx &c, bool potential);  // sorted -> caller order

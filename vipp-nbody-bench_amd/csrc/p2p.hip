@@ -393,8 +393,36 @@ __global__ __launch_bounds__(256) void k_p2p_items(const int32_t *__restrict__ l
     for (uint32_t q = 0; q < nch; ++q) items[pos + q] = make_int2(k, static_cast<int>(q));
 }
 
+// Work items of the P2P: counts per leaf, then the plan scalars (items per
+// class) copied back to pinned host words, with ev_plan recorded after the
+// copy.  Nothing here waits, so the caller can enqueue other work before
+// p2p_run needs the scalars.
+void p2p_plan(Ctx &c) {
+    TreeDev &t = c.t;
+    if (!t.nleaves) return;
+    const int nl = static_cast<int>(t.nleaves);
+    const int nb = static_cast<int>(ceil_div(nl, 256));
+    c.p2p_scal.ensure(16);
+    c.p2p_nchunk.ensure(nl);
+    c.p2p_item_off.ensure(nl);
+    c.p2p_local.ensure(t.ncells);
+    c.p2p_pre.ensure(static_cast<size_t>(nl) * NSC);
+    c.p2p_btot.ensure(static_cast<size_t>(nb) * NSC);
+    c.p2p_ticket.ensure(nl);
+    k_ni_local<<<ceil_div(t.ncells, 256), 256, 0, c.stream>>>(t.ncells, t.count, t.ni, t.ni_cnt, c.p2p_local);
+    k_p2p_plan<<<nb, 256, 0, c.stream>>>(t.leaf_ids, nl, t.level, t.count, t.anc, c.p2p_local, c.p2p_nchunk, c.p2p_pre, c.p2p_btot);
+    k_p2p_plan_blocks<<<1, 256, 0, c.stream>>>(c.p2p_btot, nb, c.p2p_scal);
+    VG_LAUNCH_CHECK();
+    static_assert(2 * NCLS + 2 <= 16, "plan scalars");
+    uint32_t *hs = reinterpret_cast<uint32_t *>(static_cast<char *>(c.pinned) + 128);  // 16 pinned host words
+    VG_HIP(hipMemcpyAsync(hs, c.p2p_scal.p, 16 * sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
+    VG_HIP(hipEventRecord(c.ev_plan, c.stream));
+}
+
+// The P2P itself, on the stream p2p_plan used: waits on the host for the plan
+// scalars, then launches one kernel per leaf class.
 template <class R>
-void p2p_t(Ctx &c, bool potential) {
+void p2p_run(Ctx &c, bool potential) {
     TreeDev &t = c.t;
     if (!t.nleaves) return;
     using V4 = typename Vec4<R>::T;
@@ -415,21 +443,8 @@ void p2p_t(Ctx &c, bool potential) {
     const R mu = static_cast<R>(c.mass);
     const int nl = static_cast<int>(t.nleaves);
     const int nb = static_cast<int>(ceil_div(nl, 256));
-    c.p2p_scal.ensure(16);
-    c.p2p_nchunk.ensure(nl);
-    c.p2p_item_off.ensure(nl);
-    c.p2p_local.ensure(t.ncells);
-    c.p2p_pre.ensure(static_cast<size_t>(nl) * NSC);
-    c.p2p_btot.ensure(static_cast<size_t>(nb) * NSC);
-    c.p2p_ticket.ensure(nl);
-    k_ni_local<<<ceil_div(t.ncells, 256), 256, 0, c.stream>>>(t.ncells, t.count, t.ni, t.ni_cnt, c.p2p_local);
-    k_p2p_plan<<<nb, 256, 0, c.stream>>>(t.leaf_ids, nl, t.level, t.count, t.anc, c.p2p_local, c.p2p_nchunk, c.p2p_pre, c.p2p_btot);
-    k_p2p_plan_blocks<<<1, 256, 0, c.stream>>>(c.p2p_btot, nb, c.p2p_scal);
-    VG_LAUNCH_CHECK();
-    static_assert(2 * NCLS + 2 <= 16, "plan scalars");
-    uint32_t *hs = reinterpret_cast<uint32_t *>(static_cast<char *>(c.pinned) + 128);  // 16 pinned host words
-    VG_HIP(hipMemcpyAsync(hs, c.p2p_scal.p, 16 * sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
-    VG_HIP(hipStreamSynchronize(c.stream));
+    const uint32_t *hs = reinterpret_cast<const uint32_t *>(static_cast<const char *>(c.pinned) + 128);
+    VG_HIP(hipEventSynchronize(c.ev_plan));
     const uint32_t nitems = hs[2 * NCLS], nsplit = hs[2 * NCLS + 1];
     const uint32_t *hbase = hs + NCLS;
     c.p2p_items.ensure(std::max<uint32_t>(1, nitems));
@@ -496,11 +511,18 @@ void p2p_t(Ctx &c, bool potential) {
 
 }  // namespace
 
-void fmm_p2p(Ctx &c, bool potential) {
+void fmm_p2p_plan(Ctx &c) { p2p_plan(c); }
+
+void fmm_p2p_run(Ctx &c, bool potential) {
     if (c.prec == 0)
-        p2p_t<double>(c, potential);
+        p2p_run<double>(c, potential);
     else
-        p2p_t<float>(c, potential);
+        p2p_run<float>(c, potential);
+}
+
+void fmm_p2p(Ctx &c, bool potential) {
+    fmm_p2p_plan(c);
+    fmm_p2p_run(c, potential);
 }
 
 }  // namespace vg
