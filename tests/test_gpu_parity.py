@@ -333,22 +333,36 @@ def test_split_m2l_locals_match_plain(vg, orc):
     assert (np.abs(a - b) / scale).max() < 1e-4
 
 
-@pytest.mark.parametrize("p", [3, 5, 9, 12])
-def test_plain_m2l_locals_match_f64(vg, orc, p):
-    """The FP32 VALU M2L (1, 2, 4 and 6 row groups of 32; the even ones on two
-    thread groups) gives the FP64 path's local expansions to FP32 accuracy,
-    cell by cell, on a clustered input whose blocks hold empty octants."""
-    x, _, _ = orc.problem_initialize(40000, 9, 1e-3)
-    x[:20000] *= 0.05  # a dense core inside a sparse halo
+@pytest.mark.parametrize("size,p", [("small", 3), ("small", 5), ("small", 8), ("small", 9), ("small", 11), ("small", 12),
+                                    ("large", 3), ("large", 8), ("large", 12)])
+def test_plain_m2l_locals_match_f64(vg, orc, size, p):
+    """The FP32 VALU M2L (1 to 6 row groups of 32; the even ones on two thread
+    groups) gives the FP64 path's local expansions to FP32 accuracy, cell by
+    cell.  "small": a clustered input whose blocks hold empty octants, few
+    parents, so blocks take half the targets; "large": 2^21 uniform points at
+    s = 16, 37,448 parents, so blocks take the full target count."""
+    if size == "small":
+        x, _, _ = orc.problem_initialize(40000, 9, 1e-3)
+        x[:20000] *= 0.05  # a dense core inside a sparse halo
+        s = 32
+    else:
+        x = np.random.default_rng(p).random((1 << 21, 3))
+        s = 16
     Ls = {}
     for mode in ("f64", "f32"):
-        with vg.Gravity(len(x), 32, p, mode, eps=1e-3) as gr:
+        with vg.Gravity(len(x), s, p, mode, eps=1e-3) as gr:
             gr.upload(x)
             gr.accelerations(copy=False)
             Ls[mode] = gr.expansions()[1]
+            if size == "large":
+                assert gr.counts()["cells"] > 8 * 37000
     a, b = Ls["f64"], Ls["f32"]
     scale = np.abs(a).max(axis=(1, 2), keepdims=True) + 1e-300
-    assert (np.abs(a - b) / scale).max() < 2e-5
+    rel = np.abs(a - b) / scale
+    # FP32 sums of 189 K products per coefficient: about 1e-7 typically; the
+    # maximum over the 2.4e7 coefficients of the large input reaches 2e-5
+    assert np.median(rel[np.abs(a) > 1e-6 * scale]) < 1e-6
+    assert rel.max() < (2e-5 if size == "small" else 5e-5)
 
 
 def test_energy_f64(vg, orc):

@@ -395,6 +395,7 @@ int vipp_grav_create(const vipp_grav_config *cfg, vipp_grav_ctx **out) {
             c.flags = cfg->flags;
             c.device = cfg->device;
             VG_HIP(hipSetDevice(c.device));
+            VG_HIP(hipDeviceGetAttribute(&c.ncu, hipDeviceAttributeMultiprocessorCount, c.device));
             VG_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
             VG_HIP(hipStreamCreateWithFlags(&c.stream2, hipStreamNonBlocking));
             VG_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));

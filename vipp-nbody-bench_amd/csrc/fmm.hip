@@ -877,11 +877,8 @@ void m2l_plain_launch(Ctx &c, const int32_t *parents, int np, unsigned grid) {
     VG_LAUNCH_CHECK();
 }
 
-template <class R>
-void m2l_plain_list_t(Ctx &c, const int32_t *parents, int np) {
-    const TreeDev &t = c.t;
-    if (np <= 0) return;
-    constexpr int TB = sizeof(R) == 4 ? 64 : 32;
+template <class R, int TB>
+void m2l_plain_rows(Ctx &c, const int32_t *parents, int np) {
     // FP32 with an even number of 32-row groups: two thread groups split the
     // rows (512 threads, twice the wavefronts per CU within the same LDS)
     constexpr int RG2 = sizeof(R) == 4 ? 2 : 1;
@@ -896,7 +893,20 @@ void m2l_plain_list_t(Ctx &c, const int32_t *parents, int np) {
     case 6: m2l_plain_launch<R, TB, 6, RG2>(c, parents, np, grid); break;
     default: throw std::runtime_error("plain M2L: expansion order out of range");
     }
-    (void)t;
+}
+
+template <class R>
+void m2l_plain_list_t(Ctx &c, const int32_t *parents, int np) {
+    if (np <= 0) return;
+    constexpr int TB = sizeof(R) == 4 ? 64 : 32;
+    // Fewer than four blocks per CU at TB targets (2^20 uniform points have
+    // 4,680 parents: 592 blocks): half the targets per block, twice the
+    // blocks, which keeps more wavefronts in flight per CU (FP32 p = 8 on that
+    // input: 2.97 against 3.35 ms).
+    if (8 * static_cast<int64_t>(ceil_div(np, TB)) < 4 * static_cast<int64_t>(c.ncu))
+        m2l_plain_rows<R, TB / 2>(c, parents, np);
+    else
+        m2l_plain_rows<R, TB>(c, parents, np);
 }
 
 template <class R>

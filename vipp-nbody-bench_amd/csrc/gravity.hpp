@@ -88,6 +88,7 @@ struct Ctx {
     double eps = 0, G = 1, mass = 0;
     uint32_t flags = 0;
     int device = 0;
+    int ncu = 0;  // compute units of the device
     bool has_masses = false;
     bool uploaded = false;
     bool evaluated = false;
