@@ -49,7 +49,19 @@ struct Vec4<double> {
 };
 
 __device__ inline float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
-__device__ inline double rsq(double x) { return 1.0 / sqrt(x); }
+// 1/sqrt(x) in FP64: the hardware estimate refined by two Newton steps
+// y <- y + y (1 - x y^2) / 2, each squaring the relative error, so the result
+// is within a few ulps of the correctly rounded value (x > 0; x = 0 gives NaN,
+// which the self-pair select replaces)
+__device__ inline double rsq(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const double e = fma(-x, y * y, 1.0);
+        y = fma(y * e, 0.5, y);
+    }
+    return y;
+}
 
 // Sources [src0, src1) of the leaf's flattened list against targets pt0 ..
 // pt0+nt-1.  out == nullptr: write the final G-scaled result to acc; else
