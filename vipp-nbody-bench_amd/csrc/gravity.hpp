@@ -101,6 +101,7 @@ struct Ctx {
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_lists = nullptr;
     hipEvent_t ev_plan = nullptr;  // the P2P plan scalars are in the pinned words
+    uint32_t p2p_chunk = 0;        // sources per P2P work item of the current plan
     bool p2p_pending = false;
     // pinned host words for the scalars read back during an evaluation (tree
     // level sizes, root box, P2P plan): a pinned copy returns sooner

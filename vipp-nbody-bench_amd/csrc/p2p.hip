@@ -9,9 +9,9 @@
 // of the sorted particles, so each entry is a (start, count) range.
 //
 // Work items: a plan kernel counts each non-empty leaf's sources; every leaf
-// is cut into chunks of at most P2P_CHUNK sources and each (leaf, chunk) pair
-// is one 256-thread block, so leaves with long inherited lists spread over
-// many blocks.  A block flattens its ranges in LDS, streams the chunk's sources
+// is cut into chunks of at most P2P_CHUNK sources (fewer when the leaves are
+// too few to fill the device) and each (leaf, chunk) pair is one 256-thread
+// block, so leaves with long inherited lists spread over many blocks.  A block flattens its ranges in LDS, streams the chunk's sources
 // through LDS tiles and gives each lane T = 1..6 targets, in groups of 8 to 64
 // lanes, by leaf-size class (the fewest idle lanes).
 // A leaf with one chunk writes its result; split leaves write partial sums,
@@ -26,7 +26,7 @@ namespace {
 
 constexpr int P2P_THREADS = 256;
 constexpr int MAXE = 27 * (MAXD + 1);
-constexpr uint32_t P2P_CHUNK = 8192;  // sources per work item
+constexpr uint32_t P2P_CHUNK = 8192;  // sources per work item (at most)
 
 // Lane layout of a leaf with nt targets and T targets per lane: groups of
 // TPW lanes, the smallest TPW in {8, 16, 32, 64} with TPW T >= nt (0: none).
@@ -182,7 +182,8 @@ __global__ __launch_bounds__(P2P_THREADS) void k_p2p(const int2 *__restrict__ it
                                                      const uint32_t *__restrict__ start, const uint32_t *__restrict__ count, const int32_t *__restrict__ anc,
                                                      const int32_t *__restrict__ ni, const int32_t *__restrict__ ni_cnt,
                                                      const typename Vec4<R>::T *__restrict__ xs, R eps2, R G, R mu, typename Vec4<R>::T *__restrict__ acc,
-                                                     typename Vec4<R>::T *__restrict__ partial, uint32_t pstride, uint32_t *__restrict__ ticket) {
+                                                     typename Vec4<R>::T *__restrict__ partial, uint32_t pstride, uint32_t *__restrict__ ticket,
+                                                     uint32_t chunk) {
     using V4 = typename Vec4<R>::T;
     constexpr int CH = sizeof(R) == 4 ? 1024 : 512;
     constexpr int TR = T <= TMAX ? T : 4;  // T = TMAX + 1: leaves in batches of 256 targets, 4 per lane
@@ -238,7 +239,7 @@ __global__ __launch_bounds__(P2P_THREADS) void k_p2p(const int2 *__restrict__ it
     }
     __syncthreads();
     const uint32_t total = ent_off[E];
-    const uint32_t src0 = static_cast<uint32_t>(item.y) * P2P_CHUNK, src1 = min(total, src0 + P2P_CHUNK);
+    const uint32_t src0 = static_cast<uint32_t>(item.y) * chunk, src1 = min(total, src0 + chunk);
     V4 *out = nchunk[item.x] > 1 ? partial + static_cast<size_t>(item_off[item.x] + item.y) * pstride : nullptr;
     if constexpr (T <= TMAX) {
         p2p_leaf<R, SELF_GUARD, POT, UM, T>(tile, red, ent_start, ent_off, E, src0, src1, nt, pt0, xs, eps2, G, mu, acc, out);
@@ -278,7 +279,7 @@ __global__ __launch_bounds__(P2P_THREADS) void k_p2p(const int2 *__restrict__ it
 }
 
 // Per leaf: number of sources in its NI chain, split into chunks of at most
-// P2P_CHUNK sources (bounded work per block), and its class by targets per lane.
+// chunk sources (bounded work per block), and its class by targets per lane.
 // Sources of each cell's own NI entries (the leaf's full list is its
 // ancestors' entries and its own, solver_ref.hpp:126).
 __global__ __launch_bounds__(256) void k_ni_local(int64_t ncells, const uint32_t *__restrict__ count, const int32_t *__restrict__ ni,
@@ -315,7 +316,8 @@ __device__ inline int leaf_class(uint32_t nt) {
 // the NSC components (one leaf per thread, 256 per block) and block totals.
 __global__ __launch_bounds__(256) void k_p2p_plan(const int32_t *__restrict__ leaves, int nleaves, const int8_t *__restrict__ level,
                                                   const uint32_t *__restrict__ count, const int32_t *__restrict__ anc, const uint32_t *__restrict__ local,
-                                                  uint32_t *__restrict__ nchunk, uint32_t *__restrict__ pre, uint32_t *__restrict__ btot) {
+                                                  uint32_t *__restrict__ nchunk, uint32_t *__restrict__ pre, uint32_t *__restrict__ btot,
+                                                  uint32_t chunk) {
     __shared__ uint32_t wtot[4][NSC];
     const int k = blockIdx.x * 256 + threadIdx.x;
     const unsigned lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -324,7 +326,7 @@ __global__ __launch_bounds__(256) void k_p2p_plan(const int32_t *__restrict__ le
         const int32_t c = leaves[k];
         uint32_t src = 0;
         for (int l = 0; l <= level[c]; ++l) src += local[anc[static_cast<int64_t>(c) * (MAXD + 1) + l]];
-        const uint32_t nch = max(1u, (src + P2P_CHUNK - 1) / P2P_CHUNK);
+        const uint32_t nch = max(1u, (src + chunk - 1) / chunk);
         nchunk[k] = nch;
         v[0] = nch;
         v[1 + leaf_class(count[c])] = nch;
@@ -422,7 +424,13 @@ void p2p_plan(Ctx &c) {
     c.p2p_btot.ensure(static_cast<size_t>(nb) * NSC);
     c.p2p_ticket.ensure(nl);
     k_ni_local<<<ceil_div(t.ncells, 256), 256, 0, c.stream>>>(t.ncells, t.count, t.ni, t.ni_cnt, c.p2p_local);
-    k_p2p_plan<<<nb, 256, 0, c.stream>>>(t.leaf_ids, nl, t.level, t.count, t.anc, c.p2p_local, c.p2p_nchunk, c.p2p_pre, c.p2p_btot);
+    // under 8 leaves per CU (the 16k Plummer sphere has 430): smaller chunks,
+    // so that the work items still give every CU several blocks
+    uint32_t chunk = P2P_CHUNK;
+    if (c.ncu > 0 && nl < 8 * c.ncu)
+        chunk = std::max<uint32_t>(1024, static_cast<uint32_t>(static_cast<uint64_t>(P2P_CHUNK) * nl / (8 * c.ncu)) / 256 * 256);
+    c.p2p_chunk = chunk;
+    k_p2p_plan<<<nb, 256, 0, c.stream>>>(t.leaf_ids, nl, t.level, t.count, t.anc, c.p2p_local, c.p2p_nchunk, c.p2p_pre, c.p2p_btot, chunk);
     k_p2p_plan_blocks<<<1, 256, 0, c.stream>>>(c.p2p_btot, nb, c.p2p_scal);
     VG_LAUNCH_CHECK();
     static_assert(2 * NCLS + 2 <= 16, "plan scalars");
@@ -482,7 +490,8 @@ void p2p_run(Ctx &c, bool potential) {
         const int2 *it = items + hbase[cls];
 #define VG_P2P_T(G_, P_, U_, T_)                                                                                                                \
     k_p2p<R, G_, P_, U_, T_><<<grid, P2P_THREADS, 0, c.stream>>>(it, t.leaf_ids, c.p2p_nchunk, c.p2p_item_off, t.level, t.start, t.count, t.anc, \
-                                                                 t.ni, t.ni_cnt, xs, eps2, static_cast<R>(c.G), mu, acc, partial, pstride, c.p2p_ticket)
+                                                                 t.ni, t.ni_cnt, xs, eps2, static_cast<R>(c.G), mu, acc, partial, pstride, c.p2p_ticket, \
+                                                                 c.p2p_chunk)
 #define VG_P2P(G_, P_, U_)           \
     do {                             \
         if (cls == 0)                \
