@@ -440,7 +440,10 @@ __global__ __launch_bounds__(256) void k_l2l(const int32_t *__restrict__ ids, in
 // octant blocks of a parent chunk run side by side on one XCD and share its
 // L2.  The prologue tabulates, per parent neighbour and target, the first
 // child and the mask of non-empty children, and from them the offsets that
-// hold any source; only those are walked.  A block has RG groups of 256
+// hold any source; only those are walked.  With nsplit > 1 the walked offsets
+// are cut into nsplit consecutive parts, one block each, which write partial
+// sums per target slot (parent index * 8 + octant) to part; k_m2l_sum adds
+// them in part order.  A block has RG groups of 256
 // threads; a thread owns the rows tr + 32 (rg + RG i), i < RPT / RG, and CPT
 // consecutive targets; the operator streams through LDS in chunks of KC
 // columns, double-buffered through registers (one barrier per chunk), and the
@@ -449,7 +452,8 @@ __global__ __launch_bounds__(256) void k_l2l(const int32_t *__restrict__ ids, in
 template <class R, int TB, int RPT, int RG>
 __global__ __launch_bounds__(256 * RG) void k_m2l_plain(const int32_t *__restrict__ parents, int nparents, const int32_t *__restrict__ child,
                                                    const uint32_t *__restrict__ count, const int32_t *__restrict__ nbr27, const R *__restrict__ Mx,
-                                                   R *__restrict__ Lx, const R *__restrict__ op, int K, int ldk) {
+                                                   R *__restrict__ Lx, const R *__restrict__ op, int K, int ldk, int nsplit,
+                                                   R *__restrict__ part) {
     constexpr int KC = sizeof(R) == 4 ? 16 : 8;     // operator columns per chunk
     constexpr int CPT = TB / 8;                      // targets per thread
     constexpr int VW = 16 / sizeof(R);               // elements per 16-byte LDS read
@@ -469,8 +473,10 @@ __global__ __launch_bounds__(256 * RG) void k_m2l_plain(const int32_t *__restric
     __shared__ int16_t s_o[189], s_dn[189], s_qs[189], s_live[189];
     __shared__ int s_nlive;
     const int nchunks = (nparents + TB - 1) / TB;
-    const int w = (blockIdx.x & 7) * nchunks + (blockIdx.x >> 3);  // XCD x takes work items [x W / 8, (x + 1) W / 8)
-    if (w >= 8 * nchunks) return;
+    const int per = nchunks * nsplit;                          // work items per XCD
+    const int wi = (blockIdx.x & 7) * per + (blockIdx.x >> 3);  // XCD x takes work items [x W / 8, (x + 1) W / 8)
+    if (wi >= 8 * per) return;
+    const int sp = wi % nsplit, w = wi / nsplit;  // the parts of one (chunk, octant) are neighbours
     const int chunk = w >> 3, qt = w & 7;
     const int tid = threadIdx.x, tr = tid & 31, tc = (tid >> 5) & 7, rg = tid >> 8;
     if (tid == 0) m2l_walk(qt, s_o, s_dn, s_qs);
@@ -513,6 +519,7 @@ __global__ __launch_bounds__(256 * RG) void k_m2l_plain(const int32_t *__restric
     }
     __syncthreads();
     const int nlive = s_nlive;
+    const int lo = sp * nlive / nsplit, hi = (sp + 1) * nlive / nsplit;  // this block's walked offsets
     R acc[RPL][CPT];
 #pragma unroll
     for (int i = 0; i < RPL; ++i)
@@ -555,9 +562,9 @@ __global__ __launch_bounds__(256 * RG) void k_m2l_plain(const int32_t *__restric
             for (int jj = 0; jj < CPT; ++jj) acc[i][jj] += ov * mv[jj];
         }
     };
-    if (nlive > 0) load_op(0, 0);
+    if (hi > lo) load_op(lo, 0);
     const int wv = tid >> 6, lane = tid & 63;
-    for (int li = 0; li < nlive; ++li) {
+    for (int li = lo; li < hi; ++li) {
         const int st = s_live[li], dn = s_dn[st], qs = s_qs[st];
         // gather: wave wv takes targets wv, wv + 4 RG, ..., lanes the columns
         for (int t = wv; t < TB; t += 4 * RG) {
@@ -568,7 +575,7 @@ __global__ __launch_bounds__(256 * RG) void k_m2l_plain(const int32_t *__restric
         }
         store_op(0);
         if (nkc > 1) load_op(li, KC);
-        else if (li + 1 < nlive) load_op(li + 1, 0);
+        else if (li + 1 < hi) load_op(li + 1, 0);
         __syncthreads();  // multipoles and chunk 0 are in LDS
         for (int kc = 0; kc < nkc; ++kc) {
             const int c0 = kc * KC, b = kc & 1;
@@ -582,7 +589,7 @@ __global__ __launch_bounds__(256 * RG) void k_m2l_plain(const int32_t *__restric
             if (kc + 1 < nkc) {
                 store_op(b ^ 1);
                 if (kc + 2 < nkc) load_op(li, c0 + 2 * KC);
-                else if (li + 1 < nlive) load_op(li + 1, 0);
+                else if (li + 1 < hi) load_op(li + 1, 0);
             }
             __syncthreads();  // chunk kc is read; chunk kc + 1 is in LDS
         }
@@ -591,12 +598,32 @@ __global__ __launch_bounds__(256 * RG) void k_m2l_plain(const int32_t *__restric
     for (int jj = 0; jj < CPT; ++jj) {
         const int32_t t = tgt[tc * CPT + jj];
         if (t < 0) continue;
+        R *dst = nsplit == 1 ? Lx + static_cast<size_t>(t) * ldk
+                             : part + (static_cast<size_t>(sp) * 8 * nparents + static_cast<size_t>(chunk * TB + tc * CPT + jj) * 8 + qt) * ldk;
 #pragma unroll
         for (int i = 0; i < RPL; ++i) {
             const int r = tr + 32 * (rg + RG * i);
-            if (r < K) Lx[static_cast<size_t>(t) * ldk + r] = sg[r] * acc[i][jj];
+            if (r < K) dst[r] = sg[r] * acc[i][jj];
         }
     }
+}
+
+// The nsplit partial sums of each non-empty M2L target into its local
+// expansion, in part order (one thread per target slot and coefficient).
+template <class R>
+__global__ __launch_bounds__(256) void k_m2l_sum(const int32_t *__restrict__ parents, int nparents, const int32_t *__restrict__ child,
+                                                 const uint32_t *__restrict__ count, int nsplit, const R *__restrict__ part, R *__restrict__ Lx,
+                                                 int K, int ldk) {
+    const int64_t i = blockIdx.x * 256ll + threadIdx.x;
+    if (i >= static_cast<int64_t>(8) * nparents * K) return;
+    const int64_t slot = i / K;
+    const int r = static_cast<int>(i % K);
+    const int32_t t = child[parents[slot >> 3]] + static_cast<int32_t>(slot & 7);
+    if (count[t] == 0) return;
+    const size_t stride = static_cast<size_t>(8) * nparents * ldk;
+    R v = part[static_cast<size_t>(slot) * ldk + r];
+    for (int q = 1; q < nsplit; ++q) v += part[q * stride + static_cast<size_t>(slot) * ldk + r];
+    Lx[static_cast<size_t>(t) * ldk + r] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -854,7 +881,7 @@ void upward_t(Ctx &c) {
 }
 
 template <class R, int TB, int RPT, int RG = 1>
-void m2l_plain_launch(Ctx &c, const int32_t *parents, int np, unsigned grid) {
+void m2l_plain_launch(Ctx &c, const int32_t *parents, int np, unsigned grid, int nsplit, R *part) {
     constexpr int KC = sizeof(R) == 4 ? 16 : 8;
     const size_t lds = (static_cast<size_t>(c.K) * (TB + 16 / sizeof(R)) + 2 * KC * (32 * RPT + 1)) * sizeof(R);
     // dynamic LDS above 64 KiB is allowed per kernel and device, here for the
@@ -873,7 +900,11 @@ void m2l_plain_launch(Ctx &c, const int32_t *parents, int np, unsigned grid) {
         }
     }
     const TreeDev &t = c.t;
-    k_m2l_plain<R, TB, RPT, RG><<<grid, 256 * RG, lds, c.stream>>>(parents, np, t.child, t.count, t.nbr27, Bufs<R>::M(c), Bufs<R>::L(c), Bufs<R>::op(c), c.K, c.ldk);
+    k_m2l_plain<R, TB, RPT, RG><<<grid * nsplit, 256 * RG, lds, c.stream>>>(parents, np, t.child, t.count, t.nbr27, Bufs<R>::M(c), Bufs<R>::L(c),
+                                                                           Bufs<R>::op(c), c.K, c.ldk, nsplit, part);
+    if (nsplit > 1)
+        k_m2l_sum<R><<<ceil_div(static_cast<size_t>(8) * np * c.K, 256), 256, 0, c.stream>>>(parents, np, t.child, t.count, nsplit, part,
+                                                                                          Bufs<R>::L(c), c.K, c.ldk);
     VG_LAUNCH_CHECK();
 }
 
@@ -884,13 +915,26 @@ void m2l_plain_rows(Ctx &c, const int32_t *parents, int np) {
     constexpr int RG2 = sizeof(R) == 4 ? 2 : 1;
     const int rpt = ceil_div(c.K, 32);
     const unsigned grid = 8 * ceil_div(np, TB);  // one block per (parent chunk, child octant)
+    // still under four blocks per CU: the walked offsets of a (chunk, octant)
+    // are cut into parts, one block each (at most 32 parts, so a part keeps
+    // about six of the at most 189 offsets), and a second kernel adds the
+    // partial sums (the 16k Plummer sphere at p = 4: 40 blocks, 26 parts,
+    // FP64 M2L 0.46 -> 0.05 ms; cutting grids of up to eight blocks per CU
+    // as well gained nothing on the 2^20 cube or the 4M sphere)
+    int nsplit = 1;
+    if (c.ncu > 0 && grid < 4u * static_cast<unsigned>(c.ncu)) nsplit = std::min<int>(32, static_cast<int>(ceil_div(4 * c.ncu, grid)));
+    R *part = nullptr;
+    if (nsplit > 1) {
+        c.m2l_part.ensure(static_cast<size_t>(nsplit) * 8 * np * c.ldk * sizeof(R));
+        part = reinterpret_cast<R *>(c.m2l_part.p);
+    }
     switch (rpt) {
-    case 1: m2l_plain_launch<R, TB, 1>(c, parents, np, grid); break;
-    case 2: m2l_plain_launch<R, TB, 2, RG2>(c, parents, np, grid); break;
-    case 3: m2l_plain_launch<R, TB, 3>(c, parents, np, grid); break;
-    case 4: m2l_plain_launch<R, TB, 4, RG2>(c, parents, np, grid); break;
-    case 5: m2l_plain_launch<R, TB, 5>(c, parents, np, grid); break;
-    case 6: m2l_plain_launch<R, TB, 6, RG2>(c, parents, np, grid); break;
+    case 1: m2l_plain_launch<R, TB, 1>(c, parents, np, grid, nsplit, part); break;
+    case 2: m2l_plain_launch<R, TB, 2, RG2>(c, parents, np, grid, nsplit, part); break;
+    case 3: m2l_plain_launch<R, TB, 3>(c, parents, np, grid, nsplit, part); break;
+    case 4: m2l_plain_launch<R, TB, 4, RG2>(c, parents, np, grid, nsplit, part); break;
+    case 5: m2l_plain_launch<R, TB, 5>(c, parents, np, grid, nsplit, part); break;
+    case 6: m2l_plain_launch<R, TB, 6, RG2>(c, parents, np, grid, nsplit, part); break;
     default: throw std::runtime_error("plain M2L: expansion order out of range");
     }
 }

@@ -148,6 +148,7 @@ struct Ctx {
     DevBuf<uint16_t> opsplit;   // M2L operators as BF16 hi/lo MFMA fragments
     DevBuf<uint16_t> opsplit_ml;  // the 8 M2M and 8 L2L operators, the same way
     DevBuf<float> ml_part;      // split M2M: the 8 octant products of one level before their sum
+    DevBuf<unsigned char> m2l_part;  // plain M2L cut into parts: the partial sums (bytes)
     DevBuf<uint16_t> msplit;    // multipoles as BF16 hi / lo halves (per cell ldk hi, then ldk lo)
     DevBuf<double> consts;      // A_n^m and recurrence constants (device copy)
 
