@@ -16,6 +16,9 @@ constexpr int MAXD = 21;
 // Expansion order limit: (p+1)^2 <= 169 real coefficients per cell.
 constexpr int MAXP = 12;
 constexpr int MAXK = (MAXP + 1) * (MAXP + 1);
+// Operators built per evaluation: 8 M2M, 8 L2L and the 189 M2L offsets of
+// octant 0's walk (m2l_walk below; the other octants reuse them mirrored).
+constexpr int NOPS_USED = 16 + 189;
 
 struct Error : std::runtime_error {
     int code;
@@ -101,7 +104,7 @@ __device__ inline void aabb_block_store(const double *lo, const double *hi, doub
 // the operator T_e, is octant 0's.  m2l_walk fills, per step of that order,
 // the operator index o (offset + (3,3,3) in base 7), the parent-neighbour slot
 // dn (0..26) of the mirrored neighbour and the mirrored source octant qs.
-__device__ inline void m2l_walk(int qt, int16_t *o, int16_t *dn, int16_t *qs) {
+__host__ __device__ inline void m2l_walk(int qt, int16_t *o, int16_t *dn, int16_t *qs) {
     int st = 0;
     for (int d = 0; d < 27; ++d) {
         const int dx = d % 3 - 1, dy = (d / 3) % 3 - 1, dz = d / 9 - 1;

@@ -66,8 +66,8 @@ constexpr int HSTRIDE = 2 * ((AL + 1) * (AL + 2) / 2) + 2;  // per offset: Y tab
 // (1, ry, rz): 0..7 M2M (child centre - parent centre of child octant o),
 // 8..15 L2L (their negatives), 16 + o M2L (source - target cell, o in base 7
 // holding (dx + 3, dy + 3, dz + 3)).
-__global__ __launch_bounds__(64) void k_harmonics(int nvec, double ry, double rz, int lmax, double *__restrict__ out) {
-    const int o = blockIdx.x;
+__global__ __launch_bounds__(64) void k_harmonics(const int16_t *__restrict__ sel, int nvec, double ry, double rz, int lmax, double *__restrict__ out) {
+    const int o = sel[blockIdx.x];  // the vectors any kernel reads (see operators_build)
     const int m = threadIdx.x;
     if (o >= nvec || m > lmax) return;
     double vx, vy, vz;
@@ -144,8 +144,8 @@ __device__ inline void row_of(int r, int &j, int &k, int &part) {
 // columns of degree n, order m.
 constexpr int OPS_ROWS = 4;
 template <class R>
-__global__ __launch_bounds__(128 * OPS_ROWS) void k_ops(const double *__restrict__ harm, int p, int K, R *__restrict__ op) {
-    const int blk = blockIdx.x;  // operator index
+__global__ __launch_bounds__(128 * OPS_ROWS) void k_ops(const int16_t *__restrict__ sel, const double *__restrict__ harm, int p, int K, R *__restrict__ op) {
+    const int blk = sel[blockIdx.x];  // operator index
     const int r = blockIdx.y * OPS_ROWS + static_cast<int>(threadIdx.x >> 7);
     const int t = threadIdx.x & 127;
     const bool cmaj = blk < 16;
@@ -1064,28 +1064,46 @@ void fmm_constants_init() {
     done[static_cast<size_t>(dev)] = true;
 }
 
+// The operator (and offset vector) indices that operators_build fills: the
+// 16 M2M and L2L ones, then 16 + o for the 189 M2L offsets o of octant 0's
+// walk.  A device table, written once per context.
+void ops_selection(Ctx &c) {
+    if (c.ops_sel.p) return;
+    int16_t o[189], dn[189], qs[189];
+    m2l_walk(0, o, dn, qs);
+    std::vector<int16_t> h(NOPS_USED);
+    for (int i = 0; i < 16; ++i) h[static_cast<size_t>(i)] = static_cast<int16_t>(i);
+    for (int i = 0; i < 189; ++i) h[static_cast<size_t>(16 + i)] = static_cast<int16_t>(16 + o[i]);
+    c.ops_sel.ensure(NOPS_USED);
+    VG_HIP(hipMemcpy(c.ops_sel.p, h.data(), NOPS_USED * sizeof(int16_t), hipMemcpyHostToDevice));
+}
+
 void operators_build(Ctx &c) {
     fmm_constants_init();
     const TreeDev &t = c.t;
     const double wx = t.box[3] - t.box[0], wy = t.box[4] - t.box[1], wz = t.box[5] - t.box[2];
     const double ry = wx > 0 ? wy / wx : 1.0, rz = wx > 0 ? wz / wx : 1.0;
     // 8 M2M (child - parent), 8 L2L (parent - child) and 343 M2L (source -
-    // target) offsets, formed on the device from the aspect ratio
+    // target) offsets, formed on the device from the aspect ratio.  Only the
+    // 189 M2L offsets of octant 0's walk are ever read (every octant uses them
+    // through the mirror signs), so only those and the 16 M2M / L2L operators
+    // are built; the other slots stay unwritten.
     const int nvec = 16 + 343;
+    ops_selection(c);
     c.harm.ensure(static_cast<size_t>(nvec) * HSTRIDE);
-    k_harmonics<<<nvec, 64, 0, c.stream>>>(nvec, ry, rz, 2 * c.p, c.harm);
-    dim3 grid(nvec, ceil_div(c.K, OPS_ROWS));
+    k_harmonics<<<NOPS_USED, 64, 0, c.stream>>>(c.ops_sel, nvec, ry, rz, 2 * c.p, c.harm);
+    dim3 grid(NOPS_USED, ceil_div(c.K, OPS_ROWS));
     if (c.prec == 0) {
         c.op64.ensure(static_cast<size_t>(nvec) * c.K * c.K);
-        k_ops<double><<<grid, 128 * OPS_ROWS, 0, c.stream>>>(c.harm, c.p, c.K, c.op64);
+        k_ops<double><<<grid, 128 * OPS_ROWS, 0, c.stream>>>(c.ops_sel, c.harm, c.p, c.K, c.op64);
     } else if (c.prec == 2) {
         c.op64.ensure(static_cast<size_t>(nvec) * c.K * c.K);
-        k_ops<double><<<grid, 128 * OPS_ROWS, 0, c.stream>>>(c.harm, c.p, c.K, c.op64);
+        k_ops<double><<<grid, 128 * OPS_ROWS, 0, c.stream>>>(c.ops_sel, c.harm, c.p, c.K, c.op64);
         c.op32.ensure(static_cast<size_t>(nvec) * c.K * c.K);
         m2l_split_operators(c, nvec);
     } else {
         c.op32.ensure(static_cast<size_t>(nvec) * c.K * c.K);
-        k_ops<float><<<grid, 128 * OPS_ROWS, 0, c.stream>>>(c.harm, c.p, c.K, c.op32);
+        k_ops<float><<<grid, 128 * OPS_ROWS, 0, c.stream>>>(c.ops_sel, c.harm, c.p, c.K, c.op32);
     }
     VG_LAUNCH_CHECK();
 }

@@ -151,6 +151,7 @@ struct Ctx {
     DevBuf<unsigned char> m2l_part;  // plain M2L cut into parts: the partial sums (bytes)
     DevBuf<uint16_t> msplit;    // multipoles as BF16 hi / lo halves (per cell ldk hi, then ldk lo)
     DevBuf<double> consts;      // A_n^m and recurrence constants (device copy)
+    DevBuf<int16_t> ops_sel;    // the operator indices that are built (16 M2M / L2L, then octant 0's 189 M2L offsets)
 
     // ref-numbering scratch
     DevBuf<uint32_t> slist;

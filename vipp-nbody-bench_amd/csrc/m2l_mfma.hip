@@ -39,9 +39,10 @@ __host__ __device__ constexpr int ks_of(int K) { return (K + 15) / 16; }
 // m-tiles (hi and lo) is one contiguous range.  One block per (offset, m-tile):
 // the tile's 32 rows are read row by row into LDS (coalesced), then each thread
 // writes whole 8-coefficient fragment vectors.
-__global__ __launch_bounds__(256) void k_split_ops(const double *__restrict__ op64, int K, __bf16 *__restrict__ frag, int MT, int NKS) {
+__global__ __launch_bounds__(256) void k_split_ops(const int16_t *__restrict__ sel, const double *__restrict__ op64, int K, __bf16 *__restrict__ frag,
+                                                   int MT, int NKS) {
     extern __shared__ double tile[];  // 32 rows x (16 NKS + 1)
-    const int o = blockIdx.x / MT, mt = blockIdx.x % MT;
+    const int o = sel[16 + blockIdx.x / MT] - 16, mt = blockIdx.x % MT;  // the M2L offsets of the walk only
     const int kc = 16 * NKS, W = kc + 1;
     const double *src = op64 + static_cast<int64_t>(16 + o) * K * K;
     for (int i = threadIdx.x; i < 32 * kc; i += 256) {
@@ -400,16 +401,30 @@ __global__ __launch_bounds__(64 * NG * mt_of((P + 1) * (P + 1))) void k_m2l_spli
     }
 }
 
-// M (FP32, row stride ldk) -> per cell ldk BF16 hi values then ldk lo values
-__global__ __launch_bounds__(256) void k_split_m(const float *__restrict__ M, int64_t n, int ldk, uint16_t *__restrict__ Ms) {
-    const int64_t i = blockIdx.x * 256ll + threadIdx.x;
-    if (i >= n * ldk) return;
-    const int64_t c = i / ldk, k = i % ldk;
-    const float x = M[i];
-    const __bf16 h = static_cast<__bf16>(x);
-    const __bf16 l = static_cast<__bf16>(x - static_cast<float>(h));
-    Ms[c * 2 * ldk + k] = __builtin_bit_cast(uint16_t, h);
-    Ms[c * 2 * ldk + ldk + k] = __builtin_bit_cast(uint16_t, l);
+// M (FP32, row stride ldk, a multiple of 16) -> per cell ldk BF16 hi values
+// then ldk lo values.  A block takes SPLIT_CELLS cells, a thread 4
+// coefficients at a time (one 16-byte load, two 8-byte stores); the cell and
+// the position in it come from a 32-bit division of the block-local index.
+constexpr int SPLIT_CELLS = 32;
+__global__ __launch_bounds__(256) void k_split_m(const float4 *__restrict__ M, int64_t n, int ldk, uint2 *__restrict__ Ms) {
+    const uint32_t vpr = static_cast<uint32_t>(ldk) / 4;  // 4-coefficient vectors per cell
+    const int64_t c0 = static_cast<int64_t>(blockIdx.x) * SPLIT_CELLS;
+    const uint32_t nv = static_cast<uint32_t>(min<int64_t>(SPLIT_CELLS, n - c0)) * vpr;
+    for (uint32_t t = threadIdx.x; t < nv; t += 256) {
+        const uint32_t cl = t / vpr, k = t - cl * vpr;
+        const int64_t c = c0 + cl;
+        const float4 x = M[c * vpr + k];
+        const float v[4] = {x.x, x.y, x.z, x.w};
+        uint32_t h[4], l[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const __bf16 hb = static_cast<__bf16>(v[e]);
+            h[e] = __builtin_bit_cast(uint16_t, hb);
+            l[e] = __builtin_bit_cast(uint16_t, static_cast<__bf16>(v[e] - static_cast<float>(hb)));
+        }
+        Ms[c * 2 * vpr + k] = make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
+        Ms[c * 2 * vpr + vpr + k] = make_uint2(l[0] | (l[1] << 16), l[2] | (l[3] << 16));
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -627,7 +642,8 @@ void m2l_split_operators(Ctx &c, int nvec) {
     c.opsplit.ensure(static_cast<size_t>(343 * 2 * per * 8));
     // split mode runs M2M, M2L and L2L from the fragments: no FP32 copies
     (void)nvec;
-    k_split_ops<<<343 * MT, 256, 32 * (16 * NKS + 1) * sizeof(double), c.stream>>>(c.op64, c.K, reinterpret_cast<__bf16 *>(c.opsplit.p), MT, NKS);
+    k_split_ops<<<189 * MT, 256, 32 * (16 * NKS + 1) * sizeof(double), c.stream>>>(c.ops_sel, c.op64, c.K, reinterpret_cast<__bf16 *>(c.opsplit.p),
+                                                                                 MT, NKS);
     c.opsplit_ml.ensure(static_cast<size_t>(16 * 3 * per * 8));
     k_split_ops_ml<<<ceil_div(16 * per, 256), 256, 0, c.stream>>>(c.op64, c.K, reinterpret_cast<__bf16 *>(c.opsplit_ml.p), MT, NKS);
     VG_LAUNCH_CHECK();
@@ -656,7 +672,9 @@ void fmm_m2l_split_list(Ctx &c, const int32_t *parents, int np) {
     if (np <= 0) return;
     const int64_t nc = c.t.ncells;
     c.msplit.ensure(static_cast<size_t>(nc) * 2 * c.ldk);
-    k_split_m<<<ceil_div(nc * c.ldk, 256), 256, 0, c.stream>>>(c.M32, nc, c.ldk, c.msplit);
+    if (c.ldk % 16 != 0) throw Error(1, "split M2L: the coefficient stride must be a multiple of 16");
+    k_split_m<<<ceil_div(nc, SPLIT_CELLS), 256, 0, c.stream>>>(reinterpret_cast<const float4 *>(c.M32.p), nc, c.ldk,
+                                                             reinterpret_cast<uint2 *>(c.msplit.p));
     VG_LAUNCH_CHECK();
     switch (c.p) {
         case 1: launch_split<1>(c, parents, np); break;
