@@ -11,9 +11,10 @@
 // Work items: a plan kernel counts each non-empty leaf's sources; every leaf
 // is cut into chunks of at most P2P_CHUNK sources (fewer when the leaves are
 // too few to fill the device) and each (leaf, chunk) pair is one 256-thread
-// block, so leaves with long inherited lists spread over many blocks.  A block flattens its ranges in LDS, streams the chunk's sources
-// through LDS tiles and gives each lane T = 1..6 targets, in groups of 8 to 64
-// lanes, by leaf-size class (the fewest idle lanes).
+// block, so leaves with long inherited lists spread over many blocks.  A
+// block flattens its ranges in LDS, streams the chunk's sources through LDS
+// tiles and gives each lane T = 1..6 targets, in groups of 8 to 64 lanes, by
+// leaf-size class (the fewest idle lanes).
 // A leaf with one chunk writes its result; split leaves write partial sums,
 // and the work item that finishes last adds them in chunk order
 // (deterministic).  With a nonzero
