@@ -20,6 +20,9 @@ Prints one JSON line.  Besides value (steps/s) it carries:
                 a bounded sample of one force evaluation of the same input
   force_err     relative force error against an FP64 direct sum on a sample
   vs_plain_gpu  the plain FP32 p = 12 GPU FMM on the same input
+  order_sweep   the same input at other orders (split p = 8 and 10 by
+                default): steps/s and force error, and the speed-up over
+                the plain FP32 p = 12 run
   configs       the other workloads: 16k Plummer p = 4 FP64, 2^20 uniform
                 p = 8 FP32 with per-particle masses, and the 4M Plummer from
                 the reference's own initial-condition stream
@@ -82,6 +85,9 @@ def main():
     ap.add_argument("--drift-steps", type=int, default=100)
     ap.add_argument("--no-compare-plain", dest="compare_plain", action="store_false",
                     help="skip the plain FP32 p=12 GPU FMM run reported beside the main mode")
+    ap.add_argument("--order-sweep", default="split:8,10",
+                    help="the headline input at other expansion orders, as mode:p,p;mode:p (modes plain, split, f64; 'none' skips): "
+                         "steps/s and force error, with the speed-up over the plain run at the headline order")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); multi-GPU sharding is not part of this build", file=sys.stderr)
@@ -164,6 +170,10 @@ def main():
     if args.compare_plain and mode != "f32":
         plain = plain_reference(vgm, x, v, masses, w, n, args.steps, value, err_ref)
 
+    sweep = None
+    if args.order_sweep != "none":
+        sweep = order_sweep(vgm, x, v, masses, w, parse_sweep(args.order_sweep), args.steps, err_ref, plain)
+
     configs = None
     if args.configs != "none":
         configs = {name: config_line(vgm, workloads, name) for name in args.configs.split(",") if name}
@@ -193,6 +203,7 @@ def main():
         "phases": phases,
         "cpu_baseline": cpu,
         "vs_plain_gpu": plain,
+        "order_sweep": sweep,
         "configs": configs,
         "learned": learned,
         "energy_drift": drift,
@@ -285,6 +296,36 @@ def timed_run(vgm, x, v, masses, w, mode, p, steps, warmup=1, closure=None, err_
     if per_eval["m2l"] > 0:
         out["m2l_pairs_per_s"] = cnt["m2l_live_pairs"] / (per_eval["m2l"] * 1e-3)
     return out, d
+
+
+def parse_sweep(spec):
+    """'split:8,10;plain:10' -> [("split", 8), ("split", 10), ("plain", 10)]."""
+    out = []
+    for part in spec.split(";"):
+        if not part.strip():
+            continue
+        m, ps = part.split(":")
+        if m not in ("plain", "split", "f64"):
+            raise ValueError(f"unknown mode {m!r} in --order-sweep")
+        out += [(m, int(q)) for q in ps.split(",") if q.strip()]
+    return out
+
+
+def order_sweep(vgm, x, v, masses, w, runs, steps, err_ref, plain):
+    """The headline input at other expansion orders: steps/s and force error
+    against the same FP64 direct-sum sample, and the speed-up over the plain
+    FP32 run at the headline order (the error each order buys, side by side)."""
+    idx, d = err_ref if err_ref is not None else (None, None)
+    out = []
+    for m, p in runs:
+        mode = {"plain": "f32", "split": "split", "f64": "f64"}[m]
+        r, _ = timed_run(vgm, x, v, masses, w, mode, p, steps, err_idx=idx, d=d)
+        line = {"mode": m, "p": p, "steps_per_s": r["steps_per_s"], "force_err": r.get("force_err"),
+                "phase_ms_per_eval": r["phase_ms_per_eval"]}
+        if plain is not None:
+            line["speedup_vs_plain_at_headline_p"] = r["steps_per_s"] / plain["steps_per_s"]
+        out.append(line)
+    return out
 
 
 def config_line(vgm, workloads, name):

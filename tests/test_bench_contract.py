@@ -57,3 +57,10 @@ def test_phase_rooflines_plain_and_f64_peaks():
     f64 = bench.phase_rooflines(per_eval, cnt, 1000, 4, "f64", 10.0)
     assert f32["m2l"]["bound"] == "valu" and f32["m2l"]["peak"] == 157.3
     assert f64["m2l"]["peak"] == 78.6 and f64["p2p"]["peak"] == 78.6 and f64["p2p"]["frac_of_scalar_fp32"] is None
+
+
+def test_order_sweep_spec():
+    assert bench.parse_sweep("split:8,10") == [("split", 8), ("split", 10)]
+    assert bench.parse_sweep("split:8;plain:10,12") == [("split", 8), ("plain", 10), ("plain", 12)]
+    with pytest.raises(ValueError):
+        bench.parse_sweep("bf16:8")
