@@ -933,7 +933,13 @@ void m2l_plain_rows(Ctx &c, const int32_t *parents, int np) {
     case 2: m2l_plain_launch<R, TB, 2, RG2>(c, parents, np, grid, nsplit, part); break;
     case 3: m2l_plain_launch<R, TB, 3>(c, parents, np, grid, nsplit, part); break;
     case 4: m2l_plain_launch<R, TB, 4, RG2>(c, parents, np, grid, nsplit, part); break;
-    case 5: m2l_plain_launch<R, TB, 5>(c, parents, np, grid, nsplit, part); break;
+    case 5:  // FP32: padded to 6 row groups on two thread groups (p = 11 on the 4M Plummer input: 19.9 -> 16.7 ms;
+             // padding 3 to 4 at p = 8 was slower, 3.0 -> 4.1 ms)
+        if (sizeof(R) == 4)
+            m2l_plain_launch<R, TB, 6, RG2>(c, parents, np, grid, nsplit, part);
+        else
+            m2l_plain_launch<R, TB, 5>(c, parents, np, grid, nsplit, part);
+        break;
     case 6: m2l_plain_launch<R, TB, 6, RG2>(c, parents, np, grid, nsplit, part); break;
     default: throw std::runtime_error("plain M2L: expansion order out of range");
     }
