@@ -32,6 +32,9 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 
 __host__ __device__ constexpr int mt_of(int K) { return (K + 31) / 32; }
+// M2L row tiles: an odd count of 5 (p = 11) is padded to 6, so the kernel's
+// 2 MT wavefronts spread evenly over the 4 SIMDs of a CU
+__host__ __device__ constexpr int mt_m2l(int K) { return mt_of(K) == 5 ? 6 : mt_of(K); }
 __host__ __device__ constexpr int ks_of(int K) { return (K + 15) / 16; }
 
 // FP64 operators (row-major K x K per offset) -> BF16 hi/lo A fragments of
@@ -81,12 +84,12 @@ __global__ __launch_bounds__(256) void k_split_ops(const int16_t *__restrict__ s
 // l ^ (k8 & 15), k8 = 2 ks + l / 32 (ks counted within the half), so that
 // both the row-wise stores and the fragment loads spread over the LDS banks.
 template <int P, int NT, int NG, bool KS>
-__global__ __launch_bounds__(64 * NG * mt_of((P + 1) * (P + 1))) void k_m2l_split(const int32_t *__restrict__ parents, int nparents,
+__global__ __launch_bounds__(64 * NG * mt_m2l((P + 1) * (P + 1))) void k_m2l_split(const int32_t *__restrict__ parents, int nparents,
                                                                              const int32_t *__restrict__ child, const uint32_t *__restrict__ count,
                                                                              const int32_t *__restrict__ nbr27, const uint4 *__restrict__ Ms,
                                                                              float *__restrict__ L, const bf16x8 *__restrict__ ops, int ldk) {
     constexpr int K = (P + 1) * (P + 1);
-    constexpr int MT = mt_of(K), NKS = ks_of(K);
+    constexpr int MT = mt_m2l(K), NKS = ks_of(K);
     constexpr int NTH = 64 * MT * NG;        // threads
     constexpr int NW = KS ? NT : NT / NG;    // n-tiles per wavefront
     static_assert(NT % NG == 0, "n-tiles must split evenly over the wavefronts of an m-tile");
@@ -628,7 +631,7 @@ void launch_split(Ctx &c, const int32_t *parents, int np) {
     constexpr int NT = 4, NG = 2;  // 128 targets, 12 wavefronts: 3 per SIMD
     constexpr bool KS = ks_of(K) >= 3;
     const unsigned grid = 8 * ceil_div(np, 32 * NT);  // one block per (parent chunk, child octant)
-    k_m2l_split<P, NT, NG, KS><<<grid, 64 * NG * mt_of(K), 0, c.stream>>>(parents, np, c.t.child, c.t.count, c.t.nbr27,
+    k_m2l_split<P, NT, NG, KS><<<grid, 64 * NG * mt_m2l(K), 0, c.stream>>>(parents, np, c.t.child, c.t.count, c.t.nbr27,
                                                                      reinterpret_cast<const uint4 *>(c.msplit.p), c.L32,
                                                                      reinterpret_cast<const bf16x8 *>(c.opsplit.p), c.ldk);
     VG_LAUNCH_CHECK();
@@ -637,13 +640,13 @@ void launch_split(Ctx &c, const int32_t *parents, int np) {
 }  // namespace
 
 void m2l_split_operators(Ctx &c, int nvec) {
-    const int MT = mt_of(c.K), NKS = ks_of(c.K);
+    const int MT = mt_of(c.K), NKS = ks_of(c.K), MTL = mt_m2l(c.K);
     const int64_t per = static_cast<int64_t>(MT) * NKS * 64;
-    c.opsplit.ensure(static_cast<size_t>(343 * 2 * per * 8));
+    c.opsplit.ensure(static_cast<size_t>(343 * 2 * static_cast<int64_t>(MTL) * NKS * 64 * 8));
     // split mode runs M2M, M2L and L2L from the fragments: no FP32 copies
     (void)nvec;
-    k_split_ops<<<189 * MT, 256, 32 * (16 * NKS + 1) * sizeof(double), c.stream>>>(c.ops_sel, c.op64, c.K, reinterpret_cast<__bf16 *>(c.opsplit.p),
-                                                                                 MT, NKS);
+    k_split_ops<<<189 * MTL, 256, 32 * (16 * NKS + 1) * sizeof(double), c.stream>>>(c.ops_sel, c.op64, c.K, reinterpret_cast<__bf16 *>(c.opsplit.p),
+                                                                                  MTL, NKS);
     c.opsplit_ml.ensure(static_cast<size_t>(16 * 3 * per * 8));
     k_split_ops_ml<<<ceil_div(16 * per, 256), 256, 0, c.stream>>>(c.op64, c.K, reinterpret_cast<__bf16 *>(c.opsplit_ml.p), MT, NKS);
     VG_LAUNCH_CHECK();
