@@ -32,9 +32,11 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 
 __host__ __device__ constexpr int mt_of(int K) { return (K + 31) / 32; }
-// M2L row tiles: an odd count of 5 (p = 11) is padded to 6, so the kernel's
-// 2 MT wavefronts spread evenly over the 4 SIMDs of a CU
-__host__ __device__ constexpr int mt_m2l(int K) { return mt_of(K) == 5 ? 6 : mt_of(K); }
+// M2L row tiles: the odd counts 3 (p = 8) and 5 (p = 11) are padded to 4
+// and 6, so the kernel's 2 MT wavefronts spread evenly over the 4 SIMDs of a
+// CU (the padded rows are zero; p = 8: M2L 1.47 -> 1.38 ms and 29.4 -> 31.2
+// steps/s on the 4M Plummer input, p = 11: 3.84 -> 2.56 ms)
+__host__ __device__ constexpr int mt_m2l(int K) { return (mt_of(K) == 3 || mt_of(K) == 5) ? mt_of(K) + 1 : mt_of(K); }
 __host__ __device__ constexpr int ks_of(int K) { return (K + 15) / 16; }
 
 // FP64 operators (row-major K x K per offset) -> BF16 hi/lo A fragments of
