@@ -27,7 +27,7 @@ namespace {
 
 constexpr int P2P_THREADS = 256;
 constexpr int MAXE = 27 * (MAXD + 1);
-constexpr uint32_t P2P_CHUNK = 8192;  // sources per work item (at most)
+constexpr uint32_t P2P_CHUNK = 16384;  // sources per work item (at most; 8192 and 32768 measured slower on the 4M Plummer input)
 
 // Lane layout of a leaf with nt targets and T targets per lane: groups of
 // TPW lanes, the smallest TPW in {8, 16, 32, 64} with TPW T >= nt (0: none).
@@ -425,11 +425,12 @@ void p2p_plan(Ctx &c) {
     c.p2p_btot.ensure(static_cast<size_t>(nb) * NSC);
     c.p2p_ticket.ensure(nl);
     k_ni_local<<<ceil_div(t.ncells, 256), 256, 0, c.stream>>>(t.ncells, t.count, t.ni, t.ni_cnt, c.p2p_local);
-    // under 8 leaves per CU (the 16k Plummer sphere has 430): smaller chunks,
-    // so that the work items still give every CU several blocks
+    // under 8 leaves per CU (the 16k Plummer sphere has 430): chunks of 8192
+    // sources scaled down by the leaf shortfall (at least 1024), so that the
+    // work items still give every CU several blocks
     uint32_t chunk = P2P_CHUNK;
     if (c.ncu > 0 && nl < 8 * c.ncu)
-        chunk = std::max<uint32_t>(1024, static_cast<uint32_t>(static_cast<uint64_t>(P2P_CHUNK) * nl / (8 * c.ncu)) / 256 * 256);
+        chunk = std::max<uint32_t>(1024, static_cast<uint32_t>(8192ull * nl / (8 * c.ncu)) / 256 * 256);
     c.p2p_chunk = chunk;
     k_p2p_plan<<<nb, 256, 0, c.stream>>>(t.leaf_ids, nl, t.level, t.count, t.anc, c.p2p_local, c.p2p_nchunk, c.p2p_pre, c.p2p_btot, chunk);
     k_p2p_plan_blocks<<<1, 256, 0, c.stream>>>(c.p2p_btot, nb, c.p2p_scal);
