@@ -74,7 +74,7 @@ def test_reference_different_precision(types, ref):
     """ref f32 / split: the float reference is GpuSolver<float, ...>, as the
     alias swap of solver.hpp makes it in main.cpp:34; ref cpu: it is the
     reference's own SolverRef<float, ...> on the CPU, so the GPU solver is
-    judged against the reference's arithmetic."""
+    checked against the reference's arithmetic."""
     if not os.path.exists(DROPIN):
         pytest.skip("oracle/_ref/dropin is built only where the reference checkout exists")
     env = dict(os.environ, OMP_NUM_THREADS="1")  # the reference's M2L loop is not race-free (module docstring)
